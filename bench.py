@@ -1,0 +1,189 @@
+"""Benchmark: rays/s for a 10000 x 10000 IPM magnification map on one MI355X.
+
+Workload (the headline configuration): kappa_tot = 0.45, shear = 0.5, smooth
+fraction 0.2 (kappa_star = 0.36), Kroupa masses over 0.08-1.5 M_sun, circular
+star field from seed 1 (~1.28e6 stars by the reference's sizing formula),
+source plane 10000 x 10000 pixels over half-length 25 theta_E, 10 rays per
+pixel, FMM order p = 30, leaves of <= 32 near-field stars. A "ray" is one IPM
+cell of the image-plane lattice (4.69e10 cells).
+
+A step is one map from the resident star field: quadtree build + FMM +
+shooting every cell into a zeroed map. `value` is cells per second of the
+whole step, mixed precision (FP64 anchors, FP32 leaf-relative near field and
+Horner, pixel-local FP32 polygons). Reported beside it, on the same stars and
+lattice: the same-algorithm FP64 kernel, the reference-faithful FP64 kernel
+(on a block of rows), the median relative error of the mixed map against the
+FP64 map, the kernel's roofline, and the FP64 CPU oracle on sampled rows.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rays/sec, 10k² IPM map: mixed vs FP64 at ≤1e-3 median rel err"
+C2 = dict(kappa_tot=0.45, shear=0.5, kappa_star=0.36, theta_star=1.0, mass_function="kroupa", m_solar=1.0,
+          m_lower=0.08, m_upper=1.5, light_loss=0.001, rectangular=False, approx=True, safety_scale=1.37,
+          center_y1=0.0, center_y2=0.0, half_length_y1=25.0, half_length_y2=25.0, num_pixels_y1=10000,
+          num_pixels_y2=10000, num_rays_y=10, random_seed=1, write_stars=False, write_maps=False,
+          write_histograms=False)
+P_ORDER = 30
+PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 = unpacked FP32 (packed FP32 instructions are not used)
+PEAK_PACKED_FP32_TFLOPS = 157.3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def flop_per_cell(nbar, p):
+    # near field 10 per star, far-field Horner 8 per order, 16 for the rest of the lens equation
+    return 10.0 * nbar + 8.0 * p + 16.0
+
+
+def cpu_baseline(m, p, seconds):
+    """FP64 oracle (CPU, OpenMP) on sampled lattice rows of the same map."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from common import Mirror
+    t0 = time.time()
+    mir = Mirror(O, m, p=p)
+    t_setup = time.time() - t0
+    n1, n2 = mir.num_cells
+    rng = np.random.default_rng(0)
+    rows_done = 0
+    t_shoot = 0.0
+    batch = 16
+    while t_shoot < seconds:
+        rows = np.sort(rng.choice(n2, size=batch, replace=False))
+        t0 = time.time()
+        mir.ipm_map(rows=rows)
+        t_shoot += time.time() - t0
+        rows_done += batch
+        batch = max(16, min(512, int(batch * max(1.0, (seconds - t_shoot) / max(t_shoot, 1e-3)))))
+    cells = rows_done * n1
+    return dict(value=cells / t_shoot, unit="rays/s", cores=O.lib().orc_num_threads(), kind="port",
+                sample=f"{rows_done} random lattice rows x {n1} cells of the same map, FP64 tree+FMM "
+                       f"(tree/FMM setup {t_setup:.1f} s not timed)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--ref-rows", type=int, default=4096, help="rows for the reference-faithful FP64 kernel")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the map would be sharded by "
+              f"lattice rows with one reduction of the maps, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from irs_amd.microlensing.IPM.ipm import IPM
+
+    log("[bench] C2 10000^2 IPM: generating stars")
+    m = IPM(**C2, is_double=True, precision=1, expansion_order=P_ORDER)
+    m.prepare()
+    log(f"[bench] stars {m.num_stars}, lattice {m.num_ray_threads}")
+    for w in range(args.warmup):
+        m.build()
+        m.shoot()
+        log(f"[bench] warmup {w}: shoot {m.t_shoot_cells:.3f} s")
+    step_s, shoot_s, build_s = [], [], []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        m.build()
+        t1 = time.perf_counter()
+        m.shoot()
+        t2 = time.perf_counter()
+        step_s.append(t2 - t0)
+        build_s.append(t1 - t0)
+        shoot_s.append(m.t_shoot_cells)
+        log(f"[bench] step {k}: build {t1 - t0:.3f} s, shoot {m.t_shoot_cells:.3f} s (events), step {t2 - t0:.3f} s")
+    cells = m.cells_shot
+    nbar = m.mean_near_stars
+    m.finish()
+    mixed_map = m.magnifications.copy()
+
+    out = {}
+    fp64_rate = ref_rate = err_median = err_win = None
+    if not args.no_fp64:
+        m.precision = 0
+        m.shoot()
+        fp64_rate = cells / m.t_shoot_cells
+        log(f"[bench] fp64 same-algorithm shoot {m.t_shoot_cells:.3f} s")
+        m.finish()
+        fp64_map = m.magnifications
+        e = np.abs(mixed_map - fp64_map) / np.maximum(np.abs(fp64_map), 1e-300)
+        err_median = float(np.median(e))
+        rng = np.random.default_rng(0)
+        y0, x0 = rng.integers(0, 10000 - 512, size=2)
+        err_win = float(np.median(e[y0:y0 + 512, x0:x0 + 512]))
+        out["fp64_max_rel_err"] = float(np.max(e))
+        log(f"[bench] mixed vs fp64 median rel err {err_median:.3e} (512^2 window {err_win:.3e}), max {np.max(e):.3e}")
+        # reference-faithful FP64 kernel on a block of rows in the middle of the lattice
+        n2 = m.num_ray_threads[1]
+        r0 = n2 // 2 - args.ref_rows // 2
+        m.kernel = 1
+        m.shoot(r0, r0 + args.ref_rows)
+        ref_rate = m.cells_shot / m.t_shoot_cells
+        log(f"[bench] reference fp64 kernel: {m.cells_shot} cells in {m.t_shoot_cells:.3f} s")
+        m.kernel = 0
+        m.precision = 1
+
+    ms_per_step = 1000.0 * float(np.mean(step_s))
+    value = cells / float(np.mean(step_s))
+    shoot_rate = cells / float(np.mean(shoot_s))
+    fpc = flop_per_cell(nbar, P_ORDER)
+    achieved = shoot_rate * fpc / 1e12
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": "C2 IPM 10000x10000 px, h_y=25, kappa=0.45, gamma=0.5, s=0.2, Kroupa 0.08-1.5, "
+                               "10 rays/px, p=30, seed 1",
+                   "num_stars": m.num_stars, "cells": cells, "lattice": list(m.num_ray_threads),
+                   "expansion_order": P_ORDER, "tree_levels": m.tree_levels, "mean_near_stars": nbar},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": None,
+                     "flop_per_cell": fpc, "kernel_ms": 1000.0 * float(np.mean(shoot_s)),
+                     "peak_packed_fp32": PEAK_PACKED_FP32_TFLOPS},
+        "shoot_rays_per_s": shoot_rate,
+        "build_ms": 1000.0 * float(np.mean(build_s)),
+        "fp64_rays_per_s": fp64_rate,
+        "reference_fp64_rays_per_s": ref_rate,
+        "speedup_vs_fp64": (shoot_rate / fp64_rate) if fp64_rate else None,
+        "speedup_vs_reference_fp64": (shoot_rate / ref_rate) if ref_rate else None,
+        "median_rel_err_vs_fp64": err_median,
+        "median_rel_err_vs_fp64_512_window": err_win,
+    }
+    result.update(out)
+    if not args.no_cpu:
+        log("[bench] CPU oracle baseline")
+        result["cpu_baseline"] = cpu_baseline(m, P_ORDER, args.cpu_seconds)
+    else:
+        result["cpu_baseline"] = None
+    m.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,160 @@
+// Device engine: star field, quadtree, FMM and the shooting kernels, all in HBM.
+//
+// Layout (everything structure-of-arrays, FP64 unless noted):
+//   stars      sx, sy, sm [N], permuted into tree order by build_tree
+//   nodes      flat arrays over all levels; level l occupies
+//              [level_start[l], level_start[l+1]); children of a node are four
+//              consecutive nodes starting at child0 (tree_node.cuh:151-177)
+//   lists      nbr[8], m2l[27], p2l[5] per node (tree_node.cuh:41-49)
+//   coeffs     mult, local: (p+1) complex per node (tree_node.cuh:54-56)
+//   leaves     compact tables read by the shooting kernels: centre, 1/h, the
+//              near-field star list (own + neighbours, <= 32, stride 32) and
+//              the deflection polynomial A_i = (theta^2/h) i c_i, i = 1..p
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace mlens
+{
+
+// strides (in complex entries) of the per-leaf deflection polynomials
+__host__ __device__ inline int poly_stride_d(int p) { return (p + 3) / 4 * 4; }
+__host__ __device__ inline int poly_stride_f(int p) { return p >= 2 ? (p - 1 + 3) / 4 * 4 : 0; }
+
+enum Precision
+{
+	PREC_FP64 = 0,    // every stage in FP64
+	PREC_MIXED = 1,   // FP64 anchors, FP32 leaf-relative near field / Horner / pixel-local polygons
+};
+
+enum ShootKernel
+{
+	KERNEL_FAST = 0,        // shared-corner lattice tiles, wave-uniform leaf tables
+	KERNEL_REFERENCE = 1,   // the reference's per-cell algorithm, FP64 (ipm_functions.cuh:112-230)
+};
+
+template <typename T>
+struct DeviceBuffer
+{
+	T* ptr = nullptr;
+	size_t count = 0;
+	void alloc(size_t n);
+	void free();
+	~DeviceBuffer() { free(); }
+	DeviceBuffer() = default;
+	DeviceBuffer(const DeviceBuffer&) = delete;
+	DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+};
+
+struct StarStats
+{
+	double m_tot, m2_tot, m2_ln_tot, m_min, m_max;
+};
+
+struct ShootStats
+{
+	double ms = 0;                 // kernel time (HIP events)
+	long long cells = 0;           // cells (IPM) or rays (IRS) shot
+	long long rays_received = 0;   // IRS only
+};
+
+class Engine
+{
+public:
+	Engine();
+	~Engine();
+	Engine(const Engine&) = delete;
+	Engine& operator=(const Engine&) = delete;
+
+	hipStream_t stream() const { return stream_; }
+
+	/* stars */
+	void upload_stars(const double* aos, int n);
+	void generate_stars(int n, int rectangular, double c_re, double c_im, int mass_function, double m_lower,
+		double m_upper, double m_solar, uint32_t seed);
+	void download_stars(double* aos) const;   // AoS x1, x2, m in the current (tree) order
+	StarStats star_stats();
+	int num_stars() const { return nstars_; }
+
+	/* quadtree with the reference's subdivision and interaction-list rules */
+	void build_tree(double root_half_length);
+	int tree_levels() const { return levels_; }
+	int num_nodes() const { return num_nodes_; }
+	const std::vector<int>& level_start() const { return level_start_; }
+	void export_tree(double* geom, int* info, int* lists) const;   // same layout as the oracle's export
+
+	/* FMM (P2M, M2M, L2L, M2L, P2L) and the leaf tables for shooting */
+	void fmm(int p, const LensParams& lens);
+	int expansion_order() const { return p_; }
+	void export_coeffs(int which, double* out) const;   // 0 = multipole, 1 = local
+	double mean_near_stars(const GridParams& g) const;  // lattice-weighted mean near-field count
+
+	/* maps (row-major n2 x n1, row 0 at the top) */
+	void alloc_maps(int n1, int n2, bool ipm, Precision prec, bool parities);
+	void zero_maps();
+	ShootStats shoot_ipm(const LensParams& lens, const GridParams& g, Precision prec, ShootKernel kern,
+		long long row0 = 0, long long row1 = -1);
+	ShootStats shoot_irs(const LensParams& lens, const GridParams& g, Precision prec, long long row0 = 0,
+		long long row1 = -1);
+	void add_parity_maps();
+	void download_map(int which, void* host) const;   // 0 = total, 1 = minima, 2 = saddles
+	void map_minmax(int which, double* mn, double* mx);
+	void map_histogram(int which, bool log10_of_value, int factor, long long hist_min, long long length,
+		long long* host_hist);
+
+	/* thin test entry: lens equation at arbitrary points through the tree */
+	void deflect_points(const LensParams& lens, const double* z, int n, Precision prec, double* w, double* mu);
+
+	/* the per-ray source position of IRS rows (test entry for bit-exact binning) */
+	void irs_positions(const LensParams& lens, const GridParams& g, Precision prec, long long row, double* w);
+
+	int map_pixels(int i) const { return npix_[i]; }
+	bool maps_are_fp32() const { return map_fp32_; }
+
+private:
+	void ensure_node_capacity(int n);
+	void build_leaf_tables(const LensParams& lens);
+
+	hipStream_t stream_ = nullptr;
+	hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+
+	int nstars_ = 0;
+	DeviceBuffer<double> sx_, sy_, sm_, tx_, ty_, tm_;
+	DeviceBuffer<int> star_node_, star_node_tmp_;
+
+	int levels_ = 0;
+	int num_nodes_ = 0;
+	int node_cap_ = 0;
+	std::vector<int> level_start_;
+	DeviceBuffer<double> ncx_, ncy_, nhl_;
+	DeviceBuffer<int> nlevel_, nparent_, nchild0_, nbegin_, ncount_, nnnbr_, nnm2l_, nnp2l_;
+	DeviceBuffer<int> nnbr_, nm2l_, np2l_;
+
+	int p_ = 0;
+	DeviceBuffer<double> binom_;
+	DeviceBuffer<double> mult_, local_;   // interleaved re, im
+
+	int num_leaves_ = 0;
+	DeviceBuffer<int> leaf_of_node_;
+	DeviceBuffer<double> leaf_geom_;      // cx, cy, hl, 1/hl per leaf
+	DeviceBuffer<int> leaf_near_n_;       // padded to a multiple of 4
+	DeviceBuffer<int> leaf_near_true_;    // stars actually in the list
+	DeviceBuffer<double> leaf_near_d_;    // x, y, m (FP64), stride 32 per leaf
+	DeviceBuffer<float> leaf_near_f_;     // x - cx, y - cy, m (FP32), stride 32 per leaf
+	DeviceBuffer<double> leaf_poly_d_;    // A_p..A_1 (re, im), descending, zero-padded to a multiple of 4
+	DeviceBuffer<float> leaf_poly_f_;     // A_p..A_2 (re, im), descending, zero-padded to a multiple of 4
+
+	int npix_[2] = {0, 0};
+	bool map_fp32_ = false;
+	bool map_int_ = false;
+	bool parities_ = false;
+	DeviceBuffer<char> map_, map_min_, map_sad_;
+
+	DeviceBuffer<int> scratch_i_;
+	DeviceBuffer<double> scratch_d_;
+	DeviceBuffer<long long> scratch_ll_;
+};
+
+}  // namespace mlens

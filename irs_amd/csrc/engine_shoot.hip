@@ -1,0 +1,1362 @@
+// Shooting kernels: IPM cells (ipm_functions.cuh:112-230) and IRS rays
+// (irs_functions.cuh:111-212), plus map utilities.
+//
+// Fast IPM kernel layout: a wave owns 64 corner columns (63 cells) and walks
+// up a tile of rows; each lane deflects one corner per row and gets its right
+// neighbour's corner by a lane shuffle, so every lattice corner is deflected
+// once instead of four times (the reference deflects all four corners of each
+// cell, ipm_functions.cuh:133-166). Lanes that sit in the same leaf evaluate
+// it together: the leaf index is made wave-uniform with readfirstlane, so the
+// leaf's star list and polynomial are read through scalar loads and used as
+// scalar operands.
+#include "engine.hpp"
+#include "device_math.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <type_traits>
+
+namespace mlens
+{
+
+using dev::dcx;
+
+static inline unsigned blocks_for(long long n, int threads) { return unsigned((n + threads - 1) / threads); }
+
+struct ShootArgs
+{
+	// lens
+	double kappa, gamma, theta2, kappastar;
+	int rectangular, approx, taylor_smooth, smooth_linear;
+	double corner_re, corner_im;
+	double lin_x, lin_y;   // linear part of the lens map (with -alpha_smooth folded in when it is linear)
+	// lattice: point (I, J) = (x0 + stepx I, y0 + stepy J)
+	double x0, y0, stepx, stepy;
+	long long N1, N2;
+	// source-plane map
+	double cyx, cyy, h1, h2, s1, s2, factor;
+	int n1, n2;
+	// tree
+	const double *ncx, *ncy, *nhl;
+	const int *nchild0, *nbegin, *ncount, *nnnbr, *nnbr, *leaf_of_node;
+	const double* local;
+	int p;
+	const double *sx, *sy, *sm;
+	// leaf tables
+	const double* leaf_geom;
+	const int* leaf_near_n;
+	const double* near_d;
+	const float* near_f;
+	const double* poly_d;
+	const float* poly_f;
+	// maps
+	void *map, *mmin, *msad;
+};
+
+// The hot read-only tables travel as separate __restrict__ kernel parameters:
+// the compiler then knows that the map atomics cannot modify them, so their
+// wave-uniform reads become scalar (SMEM) loads through the scalar cache.
+#define MLENS_TABLE_PARAMS                                                                              \
+	const double* __restrict__ ncx, const double* __restrict__ ncy, const double* __restrict__ nhl,      \
+		const int* __restrict__ nchild0, const int* __restrict__ leaf_of_node,                           \
+		const double* __restrict__ leaf_geom, const int* __restrict__ leaf_near_n,                       \
+		const double* __restrict__ near_d, const float* __restrict__ near_f,                             \
+		const double* __restrict__ poly_d, const float* __restrict__ poly_f
+#define MLENS_TABLE_ARGS(a) \
+	a.ncx, a.ncy, a.nhl, a.nchild0, a.leaf_of_node, a.leaf_geom, a.leaf_near_n, a.near_d, a.near_f, a.poly_d, a.poly_f
+#define MLENS_TABLE_REBIND(a)   \
+	a.ncx = ncx;                \
+	a.ncy = ncy;                \
+	a.nhl = nhl;                \
+	a.nchild0 = nchild0;        \
+	a.leaf_of_node = leaf_of_node; \
+	a.leaf_geom = leaf_geom;    \
+	a.leaf_near_n = leaf_near_n; \
+	a.near_d = near_d;          \
+	a.near_f = near_f;          \
+	a.poly_d = poly_d;          \
+	a.poly_f = poly_f;
+
+__device__ __forceinline__ int quadrant(double x, double y, double cx, double cy)
+{
+	if (y >= cy) return (x >= cx) ? 0 : 1;
+	return (x < cx) ? 2 : 3;
+}
+
+// tree_node.cuh:529-561
+__device__ __forceinline__ int find_node(const ShootArgs& a, double x, double y)
+{
+	int node = 0;
+	while (true)
+	{
+		int c0 = a.nchild0[node];
+		if (c0 < 0) return node;
+		double cx = a.ncx[node], cy = a.ncy[node], h = a.nhl[node];
+		if (fabs(x - cx) > h || fabs(y - cy) > h) return node;
+		node = c0 + quadrant(x, y, cx, cy);
+	}
+}
+
+struct Deflection
+{
+	double wx, wy, mu;
+};
+
+/* lens equation at a leaf (leaf index L is wave-uniform) */
+template <int PREC, bool MU>
+__device__ __forceinline__ Deflection deflect_leaf(const ShootArgs& a, int L, double zx, double zy)
+{
+	const double* g = a.leaf_geom + 4 * (long long)L;
+	const double cx = g[0], cy = g[1], ih = g[3];
+	const int nn = a.leaf_near_n[L];
+	const int p = a.p;
+	const double dzx = (zx - cx) * ih, dzy = (zy - cy) * ih;
+	double ax = 0, ay = 0, px = 0, py = 0;
+	if (PREC == PREC_FP64 || MU)
+	{
+		const double* s = a.near_d + (long long)L * MAX_NUM_STARS_DIRECT * 3;
+		for (int k0 = 0; k0 < nn; k0 += 4)
+		{
+#pragma unroll
+			for (int k = k0; k < k0 + 4; k++)
+			{
+				double dx = zx - s[3 * k], dy = zy - s[3 * k + 1];
+				double f = s[3 * k + 2] * dev::rcp64(dx * dx + dy * dy);
+				ax = fma(f, dx, ax);
+				ay = fma(f, dy, ay);
+			}
+		}
+		const int SD = poly_stride_d(p);
+		const double* A = a.poly_d + (long long)L * SD * 2;
+		for (int j0 = 0; j0 < SD; j0 += 4)
+		{
+#pragma unroll
+			for (int j = j0; j < j0 + 4; j++)
+			{
+				double nx = fma(px, dzx, fma(-py, dzy, A[2 * j]));
+				double ny = fma(px, dzy, fma(py, dzx, A[2 * j + 1]));
+				px = nx;
+				py = ny;
+			}
+		}
+	}
+	else
+	{
+		// FP32 in leaf-relative coordinates; the leaf-constant term A_1 stays FP64
+		const float zrx = float(zx - cx), zry = float(zy - cy);
+		const float4* s = reinterpret_cast<const float4*>(a.near_f) + (long long)L * MAX_NUM_STARS_DIRECT;
+		float fx = 0, fy = 0;
+		for (int k0 = 0; k0 < nn; k0 += 4)
+		{
+#pragma unroll
+			for (int k = k0; k < k0 + 4; k++)
+			{
+				float4 v = s[k];
+				float dx = zrx - v.x, dy = zry - v.y;
+				float f = v.z * dev::rcp32(dx * dx + dy * dy);
+				fx = fmaf(f, dx, fx);
+				fy = fmaf(f, dy, fy);
+			}
+		}
+		ax = fx;
+		ay = fy;
+		float qx = 0, qy = 0;
+		const int SF = poly_stride_f(p);
+		const float* A = a.poly_f + (long long)L * SF * 2;
+		const float fdx = float(dzx), fdy = float(dzy);
+		for (int j0 = 0; j0 < SF; j0 += 4)
+		{
+#pragma unroll
+			for (int j = j0; j < j0 + 4; j++)
+			{
+				float nx = fmaf(qx, fdx, fmaf(-qy, fdy, A[2 * j]));
+				float ny = fmaf(qx, fdy, fmaf(qy, fdx, A[2 * j + 1]));
+				qx = nx;
+				qy = ny;
+			}
+		}
+		if (p >= 1)
+		{
+			const int SD = poly_stride_d(p);
+			const double* A1 = a.poly_d + ((long long)L * SD + SD - 1) * 2;   // A_1
+			px = A1[0] + (dzx * double(qx) - dzy * double(qy));
+			py = A1[1] + (dzx * double(qy) + dzy * double(qx));
+		}
+	}
+	Deflection d;
+	d.wx = a.lin_x * zx - a.theta2 * ax - px;
+	d.wy = a.lin_y * zy - a.theta2 * ay + py;
+	d.mu = 0;
+	if (!a.smooth_linear)
+	{
+		dcx sm = dev::alpha_smooth(dcx{zx, zy}, a.kappastar, a.rectangular, a.approx, a.taylor_smooth, dcx{a.corner_re, a.corner_im});
+		d.wx -= sm.re;
+		d.wy -= sm.im;
+	}
+	if (MU)
+	{
+		// ipm_functions.cuh:67-82 with alpha_star.cuh:54-77 and alpha_local.cuh:51-69
+		const double* s = a.near_d + (long long)L * MAX_NUM_STARS_DIRECT * 3;
+		double sx = 0, sy = 0;   // sum m / (z - z_i)^2
+		for (int k = 0; k < nn; k++)
+		{
+			double dx = zx - s[3 * k], dy = zy - s[3 * k + 1];
+			double r2 = dx * dx + dy * dy;
+			double ir4 = s[3 * k + 2] == 0 ? 0.0 : 1.0 / (r2 * r2);
+			double m = s[3 * k + 2];
+			sx += m * (dx * dx - dy * dy) * ir4;
+			sy += m * (-2 * dx * dy) * ir4;
+		}
+		// d alpha_star / d zbar = conj(-theta^2 * S)
+		double dsx = -a.theta2 * sx, dsy = a.theta2 * sy;
+		// d alpha_local / d zbar = conj( (1/h) dP/d(dz) ), P(dz) = sum_{i>=1} A_i dz^(i-1)
+		double lx = 0, ly = 0;
+		{
+			const int SD = poly_stride_d(p);
+			const double* A = a.poly_d + (long long)L * SD * 2;
+			for (int j = 0; j < SD - 1; j++)
+			{
+				double deg = double(SD - 1 - j);
+				double nx = lx * dzx - ly * dzy + deg * A[2 * j];
+				double ny = lx * dzy + ly * dzx + deg * A[2 * j + 1];
+				lx = nx;
+				ly = ny;
+			}
+			lx *= ih;
+			ly *= -ih;
+		}
+		double ddz;
+		dcx ddzb;
+		dev::d_alpha_smooth(dcx{zx, zy}, a.kappastar, a.rectangular, a.approx, a.taylor_smooth, dcx{a.corner_re, a.corner_im}, ddz, ddzb);
+		double dwdz = (1 - a.kappa) - ddz;
+		double bx = a.gamma - dsx - lx - ddzb.re;
+		double by = -dsy - ly - ddzb.im;
+		double muinv = dwdz * dwdz - (bx * bx + by * by);
+		d.mu = 1 / muinv;
+	}
+	return d;
+}
+
+/* lens equation at a node that is not a leaf (a point outside the root box,
+   tree_node.cuh:535-540): the node's own stars and neighbours' stars directly,
+   plus its local expansion, all in FP64 */
+template <bool MU>
+__device__ Deflection deflect_node(const ShootArgs& a, int node, double zx, double zy)
+{
+	double ax = 0, ay = 0, sx = 0, sy = 0;
+	auto range = [&](int b, int n) {
+		for (int s = b; s < b + n; s++)
+		{
+			double dx = zx - a.sx[s], dy = zy - a.sy[s];
+			double r2 = dx * dx + dy * dy;
+			double f = a.sm[s] / r2;
+			ax += f * dx;
+			ay += f * dy;
+			if (MU)
+			{
+				double ir4 = 1.0 / (r2 * r2);
+				sx += a.sm[s] * (dx * dx - dy * dy) * ir4;
+				sy += a.sm[s] * (-2 * dx * dy) * ir4;
+			}
+		}
+	};
+	range(a.nbegin[node], a.ncount[node]);
+	for (int j = 0; j < a.nnnbr[node]; j++)
+	{
+		int nb = a.nnbr[8 * node + j];
+		range(a.nbegin[nb], a.ncount[nb]);
+	}
+	const double h = a.nhl[node];
+	const double dzx = (zx - a.ncx[node]) / h, dzy = (zy - a.ncy[node]) / h;
+	double px = 0, py = 0, lx = 0, ly = 0;
+	const double* c = a.local + 2 * (long long)node * (a.p + 1);
+	for (int i = a.p; i >= 1; i--)
+	{
+		double nx = px * dzx - py * dzy + i * c[2 * i];
+		double ny = px * dzy + py * dzx + i * c[2 * i + 1];
+		px = nx;
+		py = ny;
+	}
+	px *= a.theta2 / h;
+	py *= a.theta2 / h;
+	Deflection d;
+	d.wx = a.lin_x * zx - a.theta2 * ax - px;
+	d.wy = a.lin_y * zy - a.theta2 * ay + py;
+	d.mu = 0;
+	if (!a.smooth_linear)
+	{
+		dcx s = dev::alpha_smooth(dcx{zx, zy}, a.kappastar, a.rectangular, a.approx, a.taylor_smooth, dcx{a.corner_re, a.corner_im});
+		d.wx -= s.re;
+		d.wy -= s.im;
+	}
+	if (MU)
+	{
+		for (int i = a.p; i >= 2; i--)
+		{
+			double nx = lx * dzx - ly * dzy + double(i) * (i - 1) * c[2 * i];
+			double ny = lx * dzy + ly * dzx + double(i) * (i - 1) * c[2 * i + 1];
+			lx = nx;
+			ly = ny;
+		}
+		lx *= a.theta2 / (h * h);
+		ly *= -a.theta2 / (h * h);
+		double dsx = -a.theta2 * sx, dsy = a.theta2 * sy;
+		double ddz;
+		dcx ddzb;
+		dev::d_alpha_smooth(dcx{zx, zy}, a.kappastar, a.rectangular, a.approx, a.taylor_smooth, dcx{a.corner_re, a.corner_im}, ddz, ddzb);
+		double dwdz = (1 - a.kappa) - ddz;
+		double bx = a.gamma - dsx - lx - ddzb.re;
+		double by = -dsy - ly - ddzb.im;
+		d.mu = 1 / (dwdz * dwdz - (bx * bx + by * by));
+	}
+	return d;
+}
+
+/* Deflect one point per lane. Lanes that share a leaf are evaluated together
+   with the leaf index made wave-uniform. cache_* hold the lane's last leaf and
+   a box slightly inside it; a point inside that box needs no new descent. */
+struct LeafCache
+{
+	int node = -1;
+	double x0 = 1, x1 = -1, y0 = 1, y1 = -1;
+};
+
+template <int PREC, bool MU>
+__device__ __forceinline__ Deflection deflect(const ShootArgs& a, double zx, double zy, LeafCache& c)
+{
+	if (!(zx > c.x0 && zx < c.x1 && zy > c.y0 && zy < c.y1))
+	{
+		c.node = find_node(a, zx, zy);
+		int L = a.leaf_of_node[c.node];
+		if (L >= 0)
+		{
+			const double* g = a.leaf_geom + 4 * (long long)L;
+			double hh = g[2] * (1 - 1e-12);
+			c.x0 = g[0] - hh;
+			c.x1 = g[0] + hh;
+			c.y0 = g[1] - hh;
+			c.y1 = g[1] + hh;
+		}
+		else
+		{
+			c.x0 = 1;
+			c.x1 = -1;
+			c.y0 = 1;
+			c.y1 = -1;
+		}
+	}
+	const int L = a.leaf_of_node[c.node];
+	Deflection d{0, 0, 0};
+	if (L >= 0)
+	{
+		bool pending = true;
+		while (pending)
+		{
+			int u = __builtin_amdgcn_readfirstlane(L);
+			// hide the equality L == u from the optimiser: otherwise it rewrites
+			// u as the per-lane L inside the branch and the leaf tables are read
+			// with per-lane vector loads instead of scalar loads
+			asm volatile("" : "+s"(u));
+			int same = (L == u);
+			asm volatile("" : "+v"(same));
+			if (same)
+			{
+				d = deflect_leaf<PREC, MU>(a, u, zx, zy);
+				pending = false;
+			}
+		}
+	}
+	else
+	{
+		d = deflect_node<MU>(a, c.node, zx, zy);
+	}
+	return d;
+}
+
+/* ---------------------------------------------------------------------------
+Triangle to pixels (polygon.cuh:349-467 semantics: the triangle's image-plane
+area `factor` is spread over the pixels in proportion to the overlap, the
+part outside the map is dropped, whole_area is taken before clipping).
+--------------------------------------------------------------------------- */
+/*
+Per-lane accumulation of map contributions. Consecutive lattice rows of one
+lane land in the same pixel most of the time, so contributions are summed in
+two register slots (most recent first) and only an evicted slot becomes a
+global atomic. On a multi-XCD MI355X device-scope atomics are carried out
+beyond the per-XCD L2 caches, so every avoided atomic is a memory round trip.
+The key is pixel index * 2 + map (0 = total or minima, 1 = saddles).
+*/
+template <typename MapT>
+struct PixelAcc
+{
+	long long k0 = -1, k1 = -1;
+	MapT v0 = 0, v1 = 0;
+};
+
+template <typename MapT, bool PARITY>
+__device__ __forceinline__ void acc_flush_one(const ShootArgs& a, long long key, MapT v)
+{
+	if (key < 0) return;
+	void* map = PARITY ? ((key & 1) ? a.msad : a.mmin) : a.map;
+	if constexpr (std::is_same<MapT, int>::value)
+		atomicAdd(reinterpret_cast<int*>(map) + (key >> 1), v);
+	else
+		unsafeAtomicAdd(reinterpret_cast<MapT*>(map) + (key >> 1), v);
+}
+
+template <typename MapT, bool PARITY>
+__device__ __forceinline__ void acc_add(const ShootArgs& a, PixelAcc<MapT>& c, long long key, MapT v)
+{
+	if (key == c.k0)
+	{
+		c.v0 += v;
+	}
+	else if (key == c.k1)
+	{
+		MapT t = c.v1 + v;
+		c.k1 = c.k0;
+		c.v1 = c.v0;
+		c.k0 = key;
+		c.v0 = t;
+	}
+	else
+	{
+		acc_flush_one<MapT, PARITY>(a, c.k1, c.v1);
+		c.k1 = c.k0;
+		c.v1 = c.v0;
+		c.k0 = key;
+		c.v0 = v;
+	}
+}
+
+template <typename MapT, bool PARITY>
+__device__ __forceinline__ void acc_flush(const ShootArgs& a, PixelAcc<MapT>& c)
+{
+	acc_flush_one<MapT, PARITY>(a, c.k0, c.v0);
+	acc_flush_one<MapT, PARITY>(a, c.k1, c.v1);
+	c.k0 = c.k1 = -1;
+	c.v0 = c.v1 = 0;
+}
+
+// general triangle: every pixel of its bounding box, areas by Green's theorem (FP64)
+template <typename MapT, bool PARITY>
+__device__ void raster_general(const ShootArgs& a, PixelAcc<MapT>& acc, int map_bit, double p0x, double p0y,
+	double p1x, double p1y, double p2x, double p2y, int X0, int X1, int Y0, int Y1)
+{
+	using R = double;
+	dev::Tri<R> t;
+	t.x[0] = p0x - X0;
+	t.y[0] = p0y - Y0;
+	t.x[1] = p1x - X0;
+	t.y[1] = p1y - Y0;
+	t.x[2] = p2x - X0;
+	t.y[2] = p2y - Y0;
+	R area2 = (t.x[1] - t.x[0]) * (t.y[2] - t.y[0]) - (t.x[2] - t.x[0]) * (t.y[1] - t.y[0]);
+	if (area2 == 0) return;
+	if (area2 < 0)
+	{
+		R sx = t.x[1], sy = t.y[1];
+		t.x[1] = t.x[2];
+		t.y[1] = t.y[2];
+		t.x[2] = sx;
+		t.y[2] = sy;
+		area2 = -area2;
+	}
+	dev::tri_setup<R>(t);
+	const R scale = a.factor * dev::recip<R>(area2 * R(0.5));
+	for (int X = X0; X <= X1; X++)
+	{
+		R ylo, yhi;
+		dev::tri_strip_yrange<R>(t, R(X - X0), R(X - X0 + 1), ylo, yhi);
+		if (!(ylo <= yhi)) continue;
+		ylo = ylo > R(-1) ? ylo : R(-1);
+		yhi = yhi < R(Y1 - Y0 + 1) ? yhi : R(Y1 - Y0 + 1);
+		int Ya = max(Y0, Y0 + int(floor(ylo)));
+		int Yb = min(Y1, Y0 + int(floor(yhi)));
+		for (int Y = Ya; Y <= Yb; Y++)
+		{
+			R ar = dev::tri_pixel_area<R>(t, R(X - X0), R(Y - Y0));
+			if (ar > 0) acc_add<MapT, PARITY>(a, acc, ((long long)a.n1 * Y + X) * 2 + map_bit, MapT(scale * ar));
+		}
+	}
+}
+
+/*
+One triangle of a cell (polygon.cuh:349-467 semantics: its image-plane area
+`factor` is spread over the pixels in proportion to the overlap; the part
+outside the map is dropped; the whole area is taken before clipping).
+Fast cases: inside one pixel (everything to that pixel), or cut by a single
+pixel line: the vertex alone on one side of the line x = X cuts off a triangle
+with area fraction t_b t_c, where t_b, t_c are the fractions of its two
+edges on its own side (similar triangles); no area is ever differenced, so
+slivers stay exact.
+*/
+template <int PREC, bool PARITY, typename MapT>
+__device__ __forceinline__ void ipm_triangle(const ShootArgs& a, PixelAcc<MapT>& acc, double p0x, double p0y, double p1x,
+	double p1y, double p2x, double p2y)
+{
+	if (!(isfinite(p0x) && isfinite(p0y) && isfinite(p1x) && isfinite(p1y) && isfinite(p2x) && isfinite(p2y))) return;
+	// polygon.cuh:116-144 (difference form) and ipm_functions.cuh:176
+	double A = ((p1y - p0y) * p0x - (p1x - p0x) * p0y + (p2y - p1y) * p1x - (p2x - p1x) * p1y + (p0y - p2y) * p2x - (p0x - p2x) * p2y) / 2;
+	if (!(fabs(A) < 10000 * a.factor) || A == 0) return;
+	const double minx = fmin(p0x, fmin(p1x, p2x)), maxx = fmax(p0x, fmax(p1x, p2x));
+	const double miny = fmin(p0y, fmin(p1y, p2y)), maxy = fmax(p0y, fmax(p1y, p2y));
+	if (minx >= a.n1 || maxx <= 0 || miny >= a.n2 || maxy <= 0) return;
+	const int map_bit = PARITY ? (A < 0 ? 0 : 1) : 0;
+	const int X0 = int(floor(fmax(minx, 0.0)));
+	const int Y0 = int(floor(fmax(miny, 0.0)));
+	const int X1 = min(a.n1 - 1, int(floor(fmin(maxx, double(a.n1)))));
+	const int Y1 = min(a.n2 - 1, int(floor(fmin(maxy, double(a.n2)))));
+	const bool inside = minx >= 0 && maxx <= a.n1 && miny >= 0 && maxy <= a.n2;
+	const MapT factor = MapT(a.factor);
+	if (inside && X0 == X1 && Y0 == Y1)
+	{
+		acc_add<MapT, PARITY>(a, acc, ((long long)a.n1 * Y0 + X0) * 2 + map_bit, factor);
+		return;
+	}
+	const bool xcut = inside && X1 == X0 + 1 && Y0 == Y1;
+	const bool ycut = inside && Y1 == Y0 + 1 && X0 == X1;
+	if (xcut || ycut)
+	{
+		// coordinate across the cut line, measured from it
+		const double L = xcut ? double(X1) : double(Y1);
+		double u0 = (xcut ? p0x : p0y) - L, u1 = (xcut ? p1x : p1y) - L, u2 = (xcut ? p2x : p2y) - L;
+		const bool s0 = u0 < 0, s1 = u1 < 0, s2 = u2 < 0;   // below/left of the line
+		const int nlow = int(s0) + int(s1) + int(s2);
+		// the lone vertex: the one whose side has a single vertex
+		const bool lone_low = (nlow == 1);
+		double ua, ub, uc;
+		if (s0 == lone_low)
+		{
+			ua = u0;
+			ub = u1;
+			uc = u2;
+		}
+		else if (s1 == lone_low)
+		{
+			ua = u1;
+			ub = u2;
+			uc = u0;
+		}
+		else
+		{
+			ua = u2;
+			ub = u0;
+			uc = u1;
+		}
+		// fractions of the edges from the lone vertex to the line
+		const double tb = ua / (ua - ub), tc = ua / (ua - uc);
+		const MapT lone = MapT(double(a.factor) * (tb * tc));
+		long long klow = (long long)a.n1 * Y0 + X0;
+		long long khigh = xcut ? klow + 1 : klow + a.n1;
+		long long klone = lone_low ? klow : khigh, kother = lone_low ? khigh : klow;
+		acc_add<MapT, PARITY>(a, acc, klone * 2 + map_bit, lone);
+		acc_add<MapT, PARITY>(a, acc, kother * 2 + map_bit, factor - lone);
+		return;
+	}
+	raster_general<MapT, PARITY>(a, acc, map_bit, p0x, p0y, p1x, p1y, p2x, p2y, X0, X1, Y0, Y1);
+}
+
+constexpr int IPM_WAVES = 4;
+constexpr int IPM_TILE_W = 63 * IPM_WAVES;
+constexpr int IPM_TILE_H = 64;
+
+template <int PREC, bool PARITY>
+__global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long long row0, long long row1,
+	MLENS_TABLE_PARAMS, void* __restrict__ map, void* __restrict__ mmin, void* __restrict__ msad)
+{
+	MLENS_TABLE_REBIND(a)
+	a.map = map;
+	a.mmin = mmin;
+	a.msad = msad;
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	const long long I = (long long)blockIdx.x * IPM_TILE_W + wave * 63 + lane;
+	const long long J0 = row0 + (long long)blockIdx.y * IPM_TILE_H;
+	if (J0 >= row1) return;
+	const long long J1 = min(J0 + IPM_TILE_H, row1);
+	const long long Ic = I < a.N1 ? I : a.N1;
+	const double zx = a.x0 + a.stepx * double(Ic);
+	const bool cell_ok = lane < 63 && I < a.N1;
+	LeafCache cache;
+	using MapT = typename std::conditional<PREC == PREC_MIXED, float, double>::type;
+	PixelAcc<MapT> acc;
+
+	auto corner = [&](long long J, double& px, double& py) {
+		double zy = a.y0 + a.stepy * double(J);
+		Deflection d = deflect<PREC, false>(a, zx, zy, cache);
+		// ipm_functions.cuh:159-165 and array_functions.cuh:70-75
+		px = ((d.wx - a.cyx) + a.h1) * a.s1;
+		py = double(a.n2) - ((d.wy - a.cyy) + a.h2) * a.s2;
+	};
+
+	double bx, by;
+	corner(J0, bx, by);
+	double brx = __shfl_down(bx, 1), bry = __shfl_down(by, 1);
+	for (long long J = J0 + 1; J <= J1; J++)
+	{
+		double tx, ty;
+		corner(J, tx, ty);
+		double trx = __shfl_down(tx, 1), try_ = __shfl_down(ty, 1);
+		if (cell_ok)
+		{
+			// cell (I, J-1): y0 = upper right, y1 = upper left, y2 = lower left, y3 = lower right
+			ipm_triangle<PREC, PARITY, MapT>(a, acc, trx, try_, tx, ty, bx, by);
+			ipm_triangle<PREC, PARITY, MapT>(a, acc, bx, by, brx, bry, trx, try_);
+		}
+		bx = tx;
+		by = ty;
+		brx = trx;
+		bry = try_;
+	}
+	acc_flush<MapT, PARITY>(a, acc);
+}
+
+/* ---------------------------------------------------------------------------
+Reference-faithful FP64 IPM kernel: one thread per cell, four corners each
+with its own tree descent, Smith divisions in alpha_star, the reference's
+polygon clipping with log/exp edge intersections (polygon.cuh:150-467).
+It is the plain-FP64 baseline the fast kernels are measured against.
+--------------------------------------------------------------------------- */
+namespace refpoly
+{
+constexpr int MAXS = 7;
+struct Poly
+{
+	double x[MAXS], y[MAXS];
+	int n;
+};
+__device__ inline void add(Poly& P, double x, double y)
+{
+	if (P.n >= MAXS)
+	{
+		P.n = 0;
+		return;
+	}
+	P.x[P.n] = x;
+	P.y[P.n] = y;
+	P.n++;
+}
+__device__ inline double area(const Poly& P)
+{
+	if (P.n < 3) return 0;
+	double r = 0;
+	for (int i = 0; i < P.n; i++)
+	{
+		int j = (i + 1) % P.n;
+		r += (P.y[j] - P.y[i]) * P.x[i] - (P.x[j] - P.x[i]) * P.y[i];
+	}
+	return r / 2;
+}
+__device__ inline double sgn(double v) { return v < -0.0 ? -1 : (v > 0 ? 1 : 0); }
+__device__ inline double yint(double X, double ax, double ay, double bx, double by)
+{
+	double dy = by - ay;
+	if (dy == 0) return ay;
+	double ldy = log(fabs(dy)), dx = bx - ax, ldx = log(fabs(dx));
+	double lt = log(fabs(X - ax)) - ldx;
+	return ay + sgn(X - ax) * sgn(dx) * sgn(dy) * exp(lt + ldy);
+}
+__device__ inline double xint(double Y, double ax, double ay, double bx, double by)
+{
+	double dx = bx - ax;
+	if (dx == 0) return ax;
+	double ldx = log(fabs(dx)), dy = by - ay, ldy = log(fabs(dy));
+	double lt = log(fabs(Y - ay)) - ldy;
+	return ax + sgn(Y - ay) * sgn(dy) * sgn(dx) * exp(lt + ldx);
+}
+// keep = the side kept in P; returns the other side
+__device__ inline void clip_x(Poly& P, double X, bool clip_left, Poly& out)
+{
+	Poly L, R;
+	L.n = R.n = 0;
+	for (int i = 0; i < P.n; i++)
+	{
+		int j = (i + 1) % P.n;
+		double ax = P.x[i], ay = P.y[i], bx = P.x[j], by = P.y[j];
+		if (ax < X)
+		{
+			add(L, ax, ay);
+			if (bx > X)
+			{
+				double y = yint(X, ax, ay, bx, by);
+				add(L, X, y);
+				add(R, X, y);
+			}
+		}
+		else if (ax > X)
+		{
+			add(R, ax, ay);
+			if (bx < X)
+			{
+				double y = yint(X, ax, ay, bx, by);
+				add(L, X, y);
+				add(R, X, y);
+			}
+		}
+		else
+		{
+			add(L, ax, ay);
+			add(R, ax, ay);
+		}
+	}
+	if (clip_left)
+	{
+		P = R;
+		out = L;
+	}
+	else
+	{
+		P = L;
+		out = R;
+	}
+}
+__device__ inline void clip_y(Poly& P, double Y, bool clip_bottom, Poly& out)
+{
+	Poly B, T;
+	B.n = T.n = 0;
+	for (int i = 0; i < P.n; i++)
+	{
+		int j = (i + 1) % P.n;
+		double ax = P.x[i], ay = P.y[i], bx = P.x[j], by = P.y[j];
+		if (ay < Y)
+		{
+			add(B, ax, ay);
+			if (by > Y)
+			{
+				double x = xint(Y, ax, ay, bx, by);
+				add(B, x, Y);
+				add(T, x, Y);
+			}
+		}
+		else if (ay > Y)
+		{
+			add(T, ax, ay);
+			if (by < Y)
+			{
+				double x = xint(Y, ax, ay, bx, by);
+				add(B, x, Y);
+				add(T, x, Y);
+			}
+		}
+		else
+		{
+			add(B, ax, ay);
+			add(T, ax, ay);
+		}
+	}
+	if (clip_bottom)
+	{
+		P = T;
+		out = B;
+	}
+	else
+	{
+		P = B;
+		out = T;
+	}
+}
+__device__ inline double minx(const Poly& P) { double m = P.x[0]; for (int i = 1; i < P.n; i++) m = fmin(m, P.x[i]); return m; }
+__device__ inline double maxx(const Poly& P) { double m = P.x[0]; for (int i = 1; i < P.n; i++) m = fmax(m, P.x[i]); return m; }
+__device__ inline double miny(const Poly& P) { double m = P.y[0]; for (int i = 1; i < P.n; i++) m = fmin(m, P.y[i]); return m; }
+__device__ inline double maxy(const Poly& P) { double m = P.y[0]; for (int i = 1; i < P.n; i++) m = fmax(m, P.y[i]); return m; }
+
+// polygon.cuh:349-467
+__device__ void allocate(Poly P, double factor, double* pixels, int n1, int n2)
+{
+	if (P.n < 3) return;
+	if (minx(P) >= n1 || maxx(P) <= 0 || miny(P) >= n2 || maxy(P) <= 0) return;
+	double whole = fabs(area(P));
+	if (whole == 0) return;
+	Poly tmp;
+	if (minx(P) < 0) clip_x(P, 0, true, tmp);
+	if (maxx(P) > n1) clip_x(P, n1, false, tmp);
+	if (miny(P) < 0) clip_y(P, 0, true, tmp);
+	if (maxy(P) > n2) clip_y(P, n2, false, tmp);
+	if (P.n < 3) return;
+	int xmin = int(minx(P)), xmax = int(maxx(P));
+	if (xmax == n1) xmax--;
+	for (int x = xmin; x < xmax; x++)
+	{
+		Poly top;
+		clip_x(P, x + 1, true, top);
+		if (top.n < 3) continue;
+		int ymin = int(miny(top)), ymax = int(maxy(top));
+		if (ymax == n2) ymax--;
+		for (int y = ymin; y < ymax; y++)
+		{
+			Poly bot;
+			clip_y(top, y + 1, true, bot);
+			if (bot.n < 3) continue;
+			unsafeAtomicAdd(&pixels[(long long)n1 * y + x], factor * fabs(area(bot)) / whole);
+		}
+		unsafeAtomicAdd(&pixels[(long long)n1 * ymax + x], factor * fabs(area(top)) / whole);
+	}
+	if (P.n < 3) return;
+	int ymin = int(miny(P)), ymax = int(maxy(P));
+	if (ymax == n2) ymax--;
+	for (int y = ymin; y < ymax; y++)
+	{
+		Poly bot;
+		clip_y(P, y + 1, true, bot);
+		if (bot.n < 3) continue;
+		unsafeAtomicAdd(&pixels[(long long)n1 * y + xmax], factor * fabs(area(bot)) / whole);
+	}
+	unsafeAtomicAdd(&pixels[(long long)n1 * ymax + xmax], factor * fabs(area(P)) / whole);
+}
+}  // namespace refpoly
+
+// ipm_functions.cuh:34-46 with alpha_star.cuh:18-41 (Smith division) and alpha_local.cuh:18-36
+__device__ Deflection deflect_reference(const ShootArgs& a, double zx, double zy)
+{
+	int node = find_node(a, zx, zy);
+	dcx as = {0, 0};
+	auto range = [&](int b, int n) {
+		for (int s = b; s < b + n; s++) as = dev::add(as, dev::cdiv(dcx{a.sm[s], 0}, dcx{zx - a.sx[s], zy - a.sy[s]}));
+	};
+	for (int j = 0; j < a.nnnbr[node]; j++)
+	{
+		int nb = a.nnbr[8 * node + j];
+		range(a.nbegin[nb], a.ncount[nb]);
+	}
+	range(a.nbegin[node], a.ncount[node]);
+	as = dev::conj(dev::scale(as, a.theta2));
+	const double h = a.nhl[node];
+	dcx dz = dcx{(zx - a.ncx[node]) / h, (zy - a.ncy[node]) / h};
+	dcx al = {0, 0};
+	const double* c = a.local + 2 * (long long)node * (a.p + 1);
+	for (int i = a.p; i >= 1; i--)
+	{
+		al = dev::mul(al, dz);
+		al = dev::add(al, dcx{c[2 * i] * i, c[2 * i + 1] * i});
+	}
+	al = dev::scale(al, a.theta2);
+	al = dev::scale(al, 1.0 / h);
+	al = dev::conj(al);
+	dcx sm = dev::alpha_smooth(dcx{zx, zy}, a.kappastar, a.rectangular, a.approx, a.taylor_smooth, dcx{a.corner_re, a.corner_im});
+	Deflection d;
+	d.wx = (1 - a.kappa) * zx + a.gamma * zx - as.re - al.re - sm.re;
+	d.wy = (1 - a.kappa) * zy - a.gamma * zy - as.im - al.im - sm.im;
+	d.mu = 0;
+	return d;
+}
+
+template <bool PARITY>
+__global__ __launch_bounds__(256) void k_ipm_reference(ShootArgs a, long long row0, long long row1)
+{
+	long long i = (long long)blockIdx.x * 16 + threadIdx.x;
+	long long j = row0 + (long long)blockIdx.y * 16 + threadIdx.y;
+	if (i >= a.N1 || j >= row1) return;
+	// ipm_functions.cuh:125-130 (a.x0, a.y0 hold center_x - half_length_x)
+	double hx = a.stepx / 2, hy = a.stepy / 2;
+	double zx = a.x0 + hx + a.stepx * double(i);
+	double zy = a.y0 + hy + a.stepy * double(j);
+	double cxs[4] = {zx + hx, zx - hx, zx - hx, zx + hx};
+	double cys[4] = {zy + hy, zy + hy, zy - hy, zy - hy};
+	double yx[4], yy[4];
+	bool bad[4];
+	for (int k = 0; k < 4; k++)
+	{
+		Deflection d = deflect_reference(a, cxs[k], cys[k]);
+		bad[k] = isnan(d.wx) || isnan(d.wy);
+		yx[k] = ((d.wx - a.cyx) + a.h1) * a.n1 / (2 * a.h1);
+		yy[k] = double(a.n2) - ((d.wy - a.cyy) + a.h2) * a.n2 / (2 * a.h2);
+	}
+	const int tris[2][3] = {{0, 1, 2}, {2, 3, 0}};
+	for (int k = 0; k < 2; k++)
+	{
+		if (bad[tris[k][0]] || bad[tris[k][1]] || bad[tris[k][2]]) continue;
+		refpoly::Poly P;
+		P.n = 0;
+		for (int v = 0; v < 3; v++) refpoly::add(P, yx[tris[k][v]], yy[tris[k][v]]);
+		double A = refpoly::area(P);
+		if (!(fabs(A) < 10000 * a.factor)) continue;
+		double* map = reinterpret_cast<double*>(PARITY ? (A < 0 ? a.mmin : a.msad) : a.map);
+		refpoly::allocate(P, a.factor, map, a.n1, a.n2);
+	}
+}
+
+/* ---------------------------------------------------------------------------
+IRS (irs_functions.cuh:111-212)
+--------------------------------------------------------------------------- */
+// irs_functions.cuh:152-184, array_functions.cuh:70-75, bit-for-bit the
+// reference's operation order; -1 when the ray misses the map
+__device__ __forceinline__ long long irs_bin(const ShootArgs& a, double wx, double wy)
+{
+#pragma clang fp contract(off)
+	double w1 = wx - a.cyx;
+	double w2 = wy - a.cyy;
+	if (w1 < -a.h1 || w1 > a.h1 || w2 < -a.h2 || w2 > a.h2) return -1;
+	int px = int((w1 + a.h1) * a.n1 / (2 * a.h1));
+	int py = int((w2 + a.h2) * a.n2 / (2 * a.h2));
+	if (px == a.n1) px--;
+	if (py == a.n2) py--;
+	py = a.n2 - 1 - py;
+	return (long long)py * a.n1 + px;
+}
+
+constexpr int IRS_THREADS = 256;
+constexpr int IRS_TILE_H = 32;
+
+template <int PREC, bool PARITY>
+__global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long row0, long long row1,
+	unsigned long long* __restrict__ received, MLENS_TABLE_PARAMS, void* __restrict__ map, void* __restrict__ mmin,
+	void* __restrict__ msad)
+{
+	MLENS_TABLE_REBIND(a)
+	a.map = map;
+	a.mmin = mmin;
+	a.msad = msad;
+	const long long i = (long long)blockIdx.x * IRS_THREADS + threadIdx.x;
+	const long long J0 = row0 + (long long)blockIdx.y * IRS_TILE_H;
+	if (J0 >= row1) return;
+	const long long J1 = min(J0 + IRS_TILE_H, row1);
+	const bool ok = i < a.N1;
+	const double zx = a.x0 + a.stepx * double(ok ? i : a.N1 - 1);
+	LeafCache cache;
+	unsigned long long got = 0;
+	// ray counts summed per lane before the atomics (see PixelAcc)
+	PixelAcc<int> tot, par;
+	for (long long j = J0; j < J1; j++)
+	{
+		double zy = a.y0 + a.stepy * double(j);
+		Deflection d = deflect<PREC, PARITY>(a, zx, zy, cache);
+		if (!ok || isnan(d.wx) || isnan(d.wy)) continue;
+		long long idx = irs_bin(a, d.wx, d.wy);
+		if (idx < 0) continue;
+		if (PARITY) acc_add<int, true>(a, par, idx * 2 + (d.mu >= 0 ? 0 : 1), 1);
+		acc_add<int, false>(a, tot, idx * 2, 1);
+		got++;
+	}
+	acc_flush<int, false>(a, tot);
+	if (PARITY) acc_flush<int, true>(a, par);
+	// wave reduction, one 64-bit atomic per wave
+	for (int off = 32; off > 0; off >>= 1) got += __shfl_down(got, off);
+	if ((threadIdx.x & 63) == 0 && got) atomicAdd(received, got);
+}
+
+template <int PREC>
+__global__ void k_irs_positions(ShootArgs a, long long row, double* w, long long* bins)
+{
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	bool ok = i < a.N1;
+	double zx = a.x0 + a.stepx * double(ok ? i : a.N1 - 1);
+	double zy = a.y0 + a.stepy * double(row);
+	LeafCache cache;
+	Deflection d = deflect<PREC, false>(a, zx, zy, cache);
+	if (!ok) return;
+	w[2 * i] = d.wx;
+	w[2 * i + 1] = d.wy;
+	bins[i] = (isnan(d.wx) || isnan(d.wy)) ? -2 : irs_bin(a, d.wx, d.wy);
+}
+
+template <int PREC, bool MU>
+__global__ void k_deflect_points(ShootArgs a, const double* z, int n, double* w, double* mu)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	bool ok = i < n;
+	double zx = ok ? z[2 * i] : 0, zy = ok ? z[2 * i + 1] : 0;
+	LeafCache cache;
+	Deflection d = deflect<PREC, MU>(a, zx, zy, cache);
+	if (!ok) return;
+	w[2 * i] = d.wx;
+	w[2 * i + 1] = d.wy;
+	if (MU) mu[i] = d.mu;
+}
+
+/* ---------------------------------------------------------------------------
+Host side
+--------------------------------------------------------------------------- */
+struct ShootArgsBuilder
+{
+	static ShootArgs make(const LensParams& L, const GridParams& g, bool irs_offset)
+	{
+		ShootArgs a;
+		std::memset(&a, 0, sizeof(a));
+		a.kappa = L.kappa;
+		a.gamma = L.gamma;
+		a.theta2 = L.theta * L.theta;
+		a.kappastar = L.kappastar;
+		a.rectangular = L.rectangular;
+		a.approx = L.approx;
+		a.taylor_smooth = L.taylor_smooth;
+		a.corner_re = L.corner_re;
+		a.corner_im = L.corner_im;
+		a.smooth_linear = (!L.rectangular && L.approx) ? 1 : 0;
+		double fold = a.smooth_linear ? L.kappastar : 0.0;   // -alpha_smooth = +kappa_star z (alpha_smooth.cuh:109-112)
+		a.lin_x = (1 - L.kappa) + L.gamma + fold;
+		a.lin_y = (1 - L.kappa) - L.gamma + fold;
+		// lattice origin: corners at (c - h) + 2 rhs I; IRS rays at ((c - h) + rhs) + 2 rhs i (irs_functions.cuh:122)
+		a.x0 = g.center_x[0] - g.half_length_x[0];
+		a.y0 = g.center_x[1] - g.half_length_x[1];
+		if (irs_offset)
+		{
+			a.x0 = a.x0 + g.ray_half_sep[0];
+			a.y0 = a.y0 + g.ray_half_sep[1];
+		}
+		a.stepx = 2 * g.ray_half_sep[0];
+		a.stepy = 2 * g.ray_half_sep[1];
+		a.N1 = g.num_cells[0];
+		a.N2 = g.num_cells[1];
+		a.cyx = g.center_y[0];
+		a.cyy = g.center_y[1];
+		a.h1 = g.half_length_y[0];
+		a.h2 = g.half_length_y[1];
+		a.n1 = g.num_pixels[0];
+		a.n2 = g.num_pixels[1];
+		a.s1 = double(a.n1) / (2 * a.h1);
+		a.s2 = double(a.n2) / (2 * a.h2);
+		// ipm_functions.cuh:170
+		a.factor = 2 * g.ray_half_sep[0] * g.ray_half_sep[1] * a.n1 * a.n2 / (2 * a.h1 * 2 * a.h2);
+		return a;
+	}
+};
+
+#define MLENS_TREE_ARGS(a)                \
+	a.ncx = ncx_.ptr;                     \
+	a.ncy = ncy_.ptr;                     \
+	a.nhl = nhl_.ptr;                     \
+	a.nchild0 = nchild0_.ptr;             \
+	a.nbegin = nbegin_.ptr;               \
+	a.ncount = ncount_.ptr;               \
+	a.nnnbr = nnnbr_.ptr;                 \
+	a.nnbr = nnbr_.ptr;                   \
+	a.leaf_of_node = leaf_of_node_.ptr;   \
+	a.local = local_.ptr;                 \
+	a.p = p_;                             \
+	a.sx = sx_.ptr;                       \
+	a.sy = sy_.ptr;                       \
+	a.sm = sm_.ptr;                       \
+	a.leaf_geom = leaf_geom_.ptr;         \
+	a.leaf_near_n = leaf_near_n_.ptr;     \
+	a.near_d = leaf_near_d_.ptr;          \
+	a.near_f = leaf_near_f_.ptr;          \
+	a.poly_d = leaf_poly_d_.ptr;          \
+	a.poly_f = leaf_poly_f_.ptr;
+
+void Engine::alloc_maps(int n1, int n2, bool ipm, Precision prec, bool parities)
+{
+	npix_[0] = n1;
+	npix_[1] = n2;
+	map_int_ = !ipm;
+	map_fp32_ = ipm && prec == PREC_MIXED;
+	parities_ = parities;
+	size_t elem = map_int_ ? sizeof(int) : (map_fp32_ ? sizeof(float) : sizeof(double));
+	size_t bytes = size_t(n1) * n2 * elem;
+	map_.alloc(bytes);
+	if (parities)
+	{
+		map_min_.alloc(bytes);
+		map_sad_.alloc(bytes);
+	}
+	zero_maps();
+}
+
+void Engine::zero_maps()
+{
+	size_t elem = map_int_ ? sizeof(int) : (map_fp32_ ? sizeof(float) : sizeof(double));
+	size_t bytes = size_t(npix_[0]) * npix_[1] * elem;
+	MLENS_HIP_CHECK(hipMemsetAsync(map_.ptr, 0, bytes, stream_));
+	if (parities_)
+	{
+		MLENS_HIP_CHECK(hipMemsetAsync(map_min_.ptr, 0, bytes, stream_));
+		MLENS_HIP_CHECK(hipMemsetAsync(map_sad_.ptr, 0, bytes, stream_));
+	}
+}
+
+ShootStats Engine::shoot_ipm(const LensParams& lens, const GridParams& g, Precision prec, ShootKernel kern, long long row0,
+	long long row1)
+{
+	// the kernels write FP32 maps in mixed precision and FP64 maps otherwise
+	if (map_int_ || map_fp32_ != (prec == PREC_MIXED && kern == KERNEL_FAST))
+		throw HipError("IPM map type does not match the requested precision", __FILE__, __LINE__);
+	ShootArgs a = ShootArgsBuilder::make(lens, g, false);
+	MLENS_TREE_ARGS(a)
+	a.map = map_.ptr;
+	a.mmin = parities_ ? map_min_.ptr : nullptr;
+	a.msad = parities_ ? map_sad_.ptr : nullptr;
+	if (row1 < 0 || row1 > a.N2) row1 = a.N2;
+	if (row0 < 0) row0 = 0;
+	ShootStats st;
+	st.cells = (row1 > row0) ? (row1 - row0) * a.N1 : 0;
+	if (st.cells == 0) return st;
+	MLENS_HIP_CHECK(hipEventRecord(ev0_, stream_));
+	if (kern == KERNEL_REFERENCE)
+	{
+		if (map_fp32_) throw HipError("the reference kernel accumulates an FP64 map", __FILE__, __LINE__);
+		dim3 grid(unsigned((a.N1 + 15) / 16), unsigned((row1 - row0 + 15) / 16));
+		dim3 block(16, 16);
+		if (parities_)
+			k_ipm_reference<true><<<grid, block, 0, stream_>>>(a, row0, row1);
+		else
+			k_ipm_reference<false><<<grid, block, 0, stream_>>>(a, row0, row1);
+	}
+	else
+	{
+		dim3 grid(unsigned((a.N1 + IPM_TILE_W - 1) / IPM_TILE_W), unsigned((row1 - row0 + IPM_TILE_H - 1) / IPM_TILE_H));
+		dim3 block(64 * IPM_WAVES);
+		if (prec == PREC_FP64)
+		{
+			if (parities_) k_ipm_fast<PREC_FP64, true><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
+			else k_ipm_fast<PREC_FP64, false><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
+		}
+		else
+		{
+			if (parities_) k_ipm_fast<PREC_MIXED, true><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
+			else k_ipm_fast<PREC_MIXED, false><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
+		}
+	}
+	MLENS_HIP_CHECK(hipGetLastError());
+	MLENS_HIP_CHECK(hipEventRecord(ev1_, stream_));
+	MLENS_HIP_CHECK(hipEventSynchronize(ev1_));
+	float ms = 0;
+	MLENS_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+	st.ms = ms;
+	return st;
+}
+
+ShootStats Engine::shoot_irs(const LensParams& lens, const GridParams& g, Precision prec, long long row0, long long row1)
+{
+	if (!map_int_) throw HipError("IRS needs an integer map", __FILE__, __LINE__);
+	ShootArgs a = ShootArgsBuilder::make(lens, g, true);
+	MLENS_TREE_ARGS(a)
+	a.map = map_.ptr;
+	a.mmin = parities_ ? map_min_.ptr : nullptr;
+	a.msad = parities_ ? map_sad_.ptr : nullptr;
+	if (row1 < 0 || row1 > a.N2) row1 = a.N2;
+	if (row0 < 0) row0 = 0;
+	ShootStats st;
+	st.cells = (row1 > row0) ? (row1 - row0) * a.N1 : 0;
+	if (st.cells == 0) return st;
+	scratch_ll_.alloc(1);
+	MLENS_HIP_CHECK(hipMemsetAsync(scratch_ll_.ptr, 0, sizeof(long long), stream_));
+	unsigned long long* rec = reinterpret_cast<unsigned long long*>(scratch_ll_.ptr);
+	dim3 grid(unsigned((a.N1 + IRS_THREADS - 1) / IRS_THREADS), unsigned((row1 - row0 + IRS_TILE_H - 1) / IRS_TILE_H));
+	MLENS_HIP_CHECK(hipEventRecord(ev0_, stream_));
+	if (prec == PREC_FP64)
+	{
+		if (parities_) k_irs_fast<PREC_FP64, true><<<grid, IRS_THREADS, 0, stream_>>>(a, row0, row1, rec, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
+		else k_irs_fast<PREC_FP64, false><<<grid, IRS_THREADS, 0, stream_>>>(a, row0, row1, rec, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
+	}
+	else
+	{
+		if (parities_) k_irs_fast<PREC_MIXED, true><<<grid, IRS_THREADS, 0, stream_>>>(a, row0, row1, rec, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
+		else k_irs_fast<PREC_MIXED, false><<<grid, IRS_THREADS, 0, stream_>>>(a, row0, row1, rec, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
+	}
+	MLENS_HIP_CHECK(hipGetLastError());
+	MLENS_HIP_CHECK(hipEventRecord(ev1_, stream_));
+	MLENS_HIP_CHECK(hipEventSynchronize(ev1_));
+	float ms = 0;
+	MLENS_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+	st.ms = ms;
+	long long got = 0;
+	MLENS_HIP_CHECK(hipMemcpy(&got, scratch_ll_.ptr, sizeof(long long), hipMemcpyDeviceToHost));
+	st.rays_received = got;
+	return st;
+}
+
+void Engine::deflect_points(const LensParams& lens, const double* z, int n, Precision prec, double* w, double* mu)
+{
+	GridParams g;
+	std::memset(&g, 0, sizeof(g));
+	g.half_length_y[0] = g.half_length_y[1] = 1;
+	g.num_pixels[0] = g.num_pixels[1] = 1;
+	ShootArgs a = ShootArgsBuilder::make(lens, g, false);
+	MLENS_TREE_ARGS(a)
+	DeviceBuffer<double> dz, dw, dmu;
+	dz.alloc(size_t(2) * n);
+	dw.alloc(size_t(2) * n);
+	dmu.alloc(n);
+	MLENS_HIP_CHECK(hipMemcpy(dz.ptr, z, sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+	if (mu)
+		k_deflect_points<PREC_FP64, true><<<blocks_for(n, 256), 256, 0, stream_>>>(a, dz.ptr, n, dw.ptr, dmu.ptr);
+	else if (prec == PREC_FP64)
+		k_deflect_points<PREC_FP64, false><<<blocks_for(n, 256), 256, 0, stream_>>>(a, dz.ptr, n, dw.ptr, nullptr);
+	else
+		k_deflect_points<PREC_MIXED, false><<<blocks_for(n, 256), 256, 0, stream_>>>(a, dz.ptr, n, dw.ptr, nullptr);
+	MLENS_HIP_CHECK(hipGetLastError());
+	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
+	MLENS_HIP_CHECK(hipMemcpy(w, dw.ptr, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+	if (mu) MLENS_HIP_CHECK(hipMemcpy(mu, dmu.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+}
+
+void Engine::irs_positions(const LensParams& lens, const GridParams& g, Precision prec, long long row, double* w)
+{
+	ShootArgs a = ShootArgsBuilder::make(lens, g, true);
+	MLENS_TREE_ARGS(a)
+	long long n = a.N1;
+	DeviceBuffer<double> dw;
+	DeviceBuffer<long long> db;
+	dw.alloc(size_t(2) * n);
+	db.alloc(n);
+	if (prec == PREC_FP64)
+		k_irs_positions<PREC_FP64><<<blocks_for(n, 256), 256, 0, stream_>>>(a, row, dw.ptr, db.ptr);
+	else
+		k_irs_positions<PREC_MIXED><<<blocks_for(n, 256), 256, 0, stream_>>>(a, row, dw.ptr, db.ptr);
+	MLENS_HIP_CHECK(hipGetLastError());
+	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
+	// layout: w[3 i] = w1, w[3 i + 1] = w2, w[3 i + 2] = GPU bin (as double)
+	std::vector<double> hw(size_t(2) * n);
+	std::vector<long long> hb(n);
+	MLENS_HIP_CHECK(hipMemcpy(hw.data(), dw.ptr, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+	MLENS_HIP_CHECK(hipMemcpy(hb.data(), db.ptr, sizeof(long long) * n, hipMemcpyDeviceToHost));
+	for (long long i = 0; i < n; i++)
+	{
+		w[3 * i] = hw[2 * i];
+		w[3 * i + 1] = hw[2 * i + 1];
+		w[3 * i + 2] = double(hb[i]);
+	}
+}
+
+double Engine::mean_near_stars(const GridParams& g) const
+{
+	// average near-field count over the lattice, weighting each leaf by the
+	// area it shares with the shooting rectangle
+	std::vector<double> geom(size_t(num_leaves_) * 4);
+	std::vector<int> nn(num_leaves_);
+	MLENS_HIP_CHECK(hipMemcpy(geom.data(), leaf_geom_.ptr, geom.size() * sizeof(double), hipMemcpyDeviceToHost));
+	MLENS_HIP_CHECK(hipMemcpy(nn.data(), leaf_near_true_.ptr, nn.size() * sizeof(int), hipMemcpyDeviceToHost));
+	double x0 = g.center_x[0] - g.half_length_x[0], x1 = g.center_x[0] + g.half_length_x[0];
+	double y0 = g.center_x[1] - g.half_length_x[1], y1 = g.center_x[1] + g.half_length_x[1];
+	double num = 0, den = 0;
+	for (int L = 0; L < num_leaves_; L++)
+	{
+		double cx = geom[4 * L], cy = geom[4 * L + 1], h = geom[4 * L + 2];
+		double ox = std::max(0.0, std::min(x1, cx + h) - std::max(x0, cx - h));
+		double oy = std::max(0.0, std::min(y1, cy + h) - std::max(y0, cy - h));
+		num += ox * oy * nn[L];
+		den += ox * oy;
+	}
+	return den > 0 ? num / den : 0;
+}
+
+/* ---------------------------------------------------------------------------
+Map utilities (array_functions.cuh:86-160)
+--------------------------------------------------------------------------- */
+template <typename V>
+__global__ void k_add3(V* total, const V* a, const V* b, long long n)
+{
+	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) total[i] = a[i] + b[i];
+}
+
+void Engine::add_parity_maps()
+{
+	if (!parities_) return;
+	long long n = (long long)npix_[0] * npix_[1];
+	if (map_int_)
+		k_add3<int><<<blocks_for(n, 256), 256, 0, stream_>>>((int*)map_.ptr, (int*)map_min_.ptr, (int*)map_sad_.ptr, n);
+	else if (map_fp32_)
+		k_add3<float><<<blocks_for(n, 256), 256, 0, stream_>>>((float*)map_.ptr, (float*)map_min_.ptr, (float*)map_sad_.ptr, n);
+	else
+		k_add3<double><<<blocks_for(n, 256), 256, 0, stream_>>>((double*)map_.ptr, (double*)map_min_.ptr, (double*)map_sad_.ptr, n);
+	MLENS_HIP_CHECK(hipGetLastError());
+	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::download_map(int which, void* host) const
+{
+	const DeviceBuffer<char>& b = which == 0 ? map_ : (which == 1 ? map_min_ : map_sad_);
+	size_t elem = map_int_ ? sizeof(int) : (map_fp32_ ? sizeof(float) : sizeof(double));
+	MLENS_HIP_CHECK(hipMemcpy(host, b.ptr, size_t(npix_[0]) * npix_[1] * elem, hipMemcpyDeviceToHost));
+}
+
+template <typename V>
+__global__ void k_minmax(const V* v, long long n, double* out)
+{
+	__shared__ double slo[256], shi[256];
+	double lo = 1e308, hi = -1e308;
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+	{
+		double x = double(v[i]);
+		lo = fmin(lo, x);
+		hi = fmax(hi, x);
+	}
+	slo[threadIdx.x] = lo;
+	shi[threadIdx.x] = hi;
+	__syncthreads();
+	for (int off = blockDim.x / 2; off > 0; off >>= 1)
+	{
+		if ((int)threadIdx.x < off)
+		{
+			slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + off]);
+			shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + off]);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+	{
+		out[2 * blockIdx.x] = slo[0];
+		out[2 * blockIdx.x + 1] = shi[0];
+	}
+}
+
+void Engine::map_minmax(int which, double* mn, double* mx)
+{
+	const DeviceBuffer<char>& b = which == 0 ? map_ : (which == 1 ? map_min_ : map_sad_);
+	long long n = (long long)npix_[0] * npix_[1];
+	const int nb = 1024;
+	scratch_d_.alloc(2 * nb);
+	if (map_int_)
+		k_minmax<int><<<nb, 256, 0, stream_>>>((const int*)b.ptr, n, scratch_d_.ptr);
+	else if (map_fp32_)
+		k_minmax<float><<<nb, 256, 0, stream_>>>((const float*)b.ptr, n, scratch_d_.ptr);
+	else
+		k_minmax<double><<<nb, 256, 0, stream_>>>((const double*)b.ptr, n, scratch_d_.ptr);
+	MLENS_HIP_CHECK(hipGetLastError());
+	std::vector<double> h(2 * nb);
+	MLENS_HIP_CHECK(hipMemcpyAsync(h.data(), scratch_d_.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
+	double lo = 1e308, hi = -1e308;
+	for (int i = 0; i < nb; i++)
+	{
+		lo = std::min(lo, h[2 * i]);
+		hi = std::max(hi, h[2 * i + 1]);
+	}
+	*mn = lo;
+	*mx = hi;
+}
+
+// array_functions.cuh:114-160: index = round(value * factor - min), or of log10(value)
+template <typename V>
+__global__ void k_histogram(const V* v, long long n, int log_mode, int factor, long long hmin, long long len,
+	unsigned long long* hist)
+{
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+	{
+		double x = double(v[i]);
+		long long idx;
+		if (log_mode)
+		{
+			if (!(x > 0)) continue;
+			idx = llround(log10(x) * factor - double(hmin));
+		}
+		else
+		{
+			idx = llround(1.0 * x * factor - double(hmin));
+		}
+		if (idx >= 0 && idx < len) atomicAdd(&hist[idx], 1ull);
+	}
+}
+
+void Engine::map_histogram(int which, bool log_mode, int factor, long long hmin, long long len, long long* host_hist)
+{
+	const DeviceBuffer<char>& b = which == 0 ? map_ : (which == 1 ? map_min_ : map_sad_);
+	long long n = (long long)npix_[0] * npix_[1];
+	DeviceBuffer<long long> dh;
+	dh.alloc(std::max(len, 1LL));
+	MLENS_HIP_CHECK(hipMemsetAsync(dh.ptr, 0, sizeof(long long) * std::max(len, 1LL), stream_));
+	unsigned long long* hp = reinterpret_cast<unsigned long long*>(dh.ptr);
+	const int nb = 2048;
+	if (map_int_)
+		k_histogram<int><<<nb, 256, 0, stream_>>>((const int*)b.ptr, n, log_mode, factor, hmin, len, hp);
+	else if (map_fp32_)
+		k_histogram<float><<<nb, 256, 0, stream_>>>((const float*)b.ptr, n, log_mode, factor, hmin, len, hp);
+	else
+		k_histogram<double><<<nb, 256, 0, stream_>>>((const double*)b.ptr, n, log_mode, factor, hmin, len, hp);
+	MLENS_HIP_CHECK(hipGetLastError());
+	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
+	MLENS_HIP_CHECK(hipMemcpy(host_hist, dh.ptr, sizeof(long long) * len, hipMemcpyDeviceToHost));
+}
+
+}  // namespace mlens

@@ -1,0 +1,242 @@
+"""GPU parity of the HIP engine against the FP64 oracle, through the C ABI.
+
+Every check runs the engine on the same star field as the oracle (the
+engine's own field, in tree order, handed to the oracle) and compares:
+  * the star field with the oracle's Philox generator (same seed),
+  * the quadtree node for node and list for list,
+  * the FMM coefficients,
+  * the lens equation at random points (tree and direct sums),
+  * whole IPM maps (fast FP64, reference-faithful FP64, mixed),
+  * IRS maps, with bit-exact binning of the engine's own ray positions.
+Tolerances are stated at each assertion.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from common import C1_SMALL, C2_SMALL, Mirror, from_starfile, random_points, rel_err, write_starfile
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_ipm(params, **extra):
+    from irs_amd.microlensing.IPM.ipm import IPM
+    kw = dict(params)
+    kw.update(extra)
+    say(f"[gpu] IPM run: {extra}")
+    m = IPM(**kw)
+    m.run()
+    return m
+
+
+@pytest.fixture(scope="module")
+def c1_stars(oracle, tmp_path_factory):
+    path = tmp_path_factory.mktemp("c1") / "c1_stars.bin"
+    stars = write_starfile(oracle, path, C1_SMALL, seed=1)
+    return path, stars
+
+
+@pytest.fixture(scope="module")
+def c1_fp64(c1_stars):
+    return run_ipm(from_starfile(C1_SMALL, c1_stars[0]), is_double=True, expansion_order=16)
+
+
+@pytest.fixture(scope="module")
+def c1_mirror(oracle, c1_fp64):
+    return Mirror(oracle, c1_fp64)
+
+
+def test_star_field_matches_philox_oracle(oracle):
+    m = run_ipm(C1_SMALL, is_double=True, expansion_order=16)
+    d = oracle.derived_params(0.3, 0.3, 0.3, 1.0, "equal", 1.0, 1.0, 1.0, 0.001, 0, 1, 1.37, (0, 0), (10, 10),
+                              (100, 100), 4)
+    assert m.num_stars == d.num_stars
+    ref = oracle.generate_stars(d.num_stars, 0, d.corner, "equal", d.m_lower, d.m_upper, 1.0, 1)
+    got = m.stars
+    a = ref[np.lexsort((ref[:, 1], ref[:, 0]))]
+    b = got[np.lexsort((got[:, 1], got[:, 0]))]
+    # same Philox stream; device and host sin/cos/sqrt may differ in the last bits
+    assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a[:, :2]))
+
+
+def test_tree_matches_oracle(c1_fp64, c1_mirror):
+    g_geom, g_info, g_lists = c1_fp64.export_tree()
+    o_geom, o_info, o_lists = c1_mirror.tree.export()
+    assert c1_fp64.tree_levels == c1_mirror.tree.levels
+    assert g_geom.shape == o_geom.shape
+    np.testing.assert_array_equal(g_geom, o_geom)   # same arithmetic for centres and half-lengths
+    np.testing.assert_array_equal(g_info, o_info)
+    np.testing.assert_array_equal(g_lists, o_lists)
+    # stars came back in the oracle's tree order
+    np.testing.assert_array_equal(c1_fp64.stars, c1_mirror.tree.stars())
+
+
+def test_fmm_coefficients_match_oracle(c1_fp64, c1_mirror):
+    for which in ("multipole", "local"):
+        g = c1_fp64.export_coeffs("local" if which == "local" else "multipole")
+        o = c1_mirror.tree.coeffs("local" if which == "local" else "multipole")
+        scale = np.maximum(np.max(np.abs(o), axis=1, keepdims=True), 1e-300)
+        err = np.max(np.abs(g - o) / scale)
+        assert err < 1e-11, (which, err)
+
+
+def test_deflection_matches_oracle(oracle, c1_fp64, c1_mirror):
+    z = random_points(c1_fp64, 20000, seed=3)
+    w_gpu = c1_fp64.deflect(z)
+    w_tree, _ = c1_mirror.tree.deflect(c1_mirror.lens, z)
+    w_direct = oracle.deflect_direct(c1_mirror.stars, c1_mirror.lens, z)
+    # same tree and expansion: FP64 rounding only
+    assert np.max(np.abs(w_gpu - w_tree)) < 1e-10
+    # the FMM itself is within the reference's error budget alpha_error (ipm.cuh:532-533)
+    assert np.max(np.abs(w_tree - w_direct)) < c1_fp64.alpha_error
+
+
+def test_magnification_matches_oracle(c1_fp64, c1_mirror):
+    z = random_points(c1_fp64, 5000, seed=4)
+    _, mu_gpu = c1_fp64.deflect(z, want_mu=True)
+    _, mu_ref, _ = c1_mirror.tree.deflect(c1_mirror.lens, z, want_mu=True)
+    assert np.median(rel_err(mu_gpu, mu_ref)) < 1e-10
+    assert np.all(np.sign(mu_gpu) == np.sign(mu_ref))
+
+
+def assert_fp64_map(got, ref, what):
+    """FP64 map against the FP64 oracle. Both evaluate the same algorithm; they
+    differ only in rounding (the device contracts a*b+c into FMAs, uses its own
+    libm and sums in another order). The map amplifies incoherent last-bit
+    differences of the corners: the oracle compiled with FMA contraction
+    differs from the stock oracle on this C1 map by median 1e-11, p99.9 8e-7,
+    max 1.9e-6. The bounds below sit just outside that envelope."""
+    assert got.shape == ref.shape
+    e = rel_err(got, ref, floor=1e-12)
+    say(f"[{what}] rel err median {np.median(e):.2e}  p99 {np.percentile(e, 99):.2e}  "
+        f"p99.9 {np.percentile(e, 99.9):.2e}  max {np.max(e):.2e}")
+    assert np.median(e) < 1e-10
+    assert np.percentile(e, 99) < 1e-6
+    assert np.max(e) < 1e-5
+    assert abs(got.sum() / ref.sum() - 1) < 1e-9   # FMA-contracted oracle: 1.39e-10
+
+
+def test_ipm_fp64_map_matches_oracle(c1_fp64, c1_mirror):
+    assert_fp64_map(c1_fp64.magnifications, c1_mirror.ipm_map(), "fast fp64")
+
+
+def test_ipm_reference_kernel_matches_oracle(c1_stars, c1_mirror):
+    m = run_ipm(from_starfile(C1_SMALL, c1_stars[0]), is_double=True, expansion_order=16, kernel=1)
+    assert_fp64_map(m.magnifications, c1_mirror.ipm_map(), "reference fp64")
+    m.close()
+
+
+def test_ipm_mixed_map_matches_oracle(oracle, c1_stars):
+    m = run_ipm(from_starfile(C1_SMALL, c1_stars[0]), is_double=False, expansion_order=16)
+    assert m.precision == 1
+    mir = Mirror(oracle, m, stars=c1_stars[1])   # the float ABI returns rounded stars; use the file's
+    ref = mir.ipm_map()
+    e = rel_err(m.magnifications, ref, floor=1e-12)
+    say(f"[mixed C1] rel err median {np.median(e):.2e}  p99 {np.percentile(e, 99):.2e}  max {np.max(e):.2e}")
+    assert np.median(e) < 1e-4, np.median(e)
+    assert np.percentile(e, 99) < 1e-2, np.percentile(e, 99)
+    m.close()
+
+
+def test_ipm_parity_maps(oracle, c1_stars, c1_mirror):
+    m = run_ipm(from_starfile(C1_SMALL, c1_stars[0]), is_double=True, expansion_order=16, write_parities=True)
+    tot, mn, sd = c1_mirror.ipm_map(parities=True)
+    assert_fp64_map(m.magnifications_minima + 1.0, mn + 1.0, "minima")   # +1: saddle-free pixels are 0
+    assert_fp64_map(m.magnifications_saddles + 1.0, sd + 1.0, "saddles")
+    assert_fp64_map(m.magnifications, tot, "minima + saddles")
+    m.close()
+
+
+def test_ipm_c2_shape_mixed_vs_fp64(oracle, tmp_path):
+    path = tmp_path / "c2_stars.bin"
+    stars = write_starfile(oracle, path, C2_SMALL, seed=7)
+    a = run_ipm(from_starfile(C2_SMALL, path), is_double=True, expansion_order=20)
+    b = run_ipm(from_starfile(C2_SMALL, path), is_double=False, expansion_order=20)
+    mir = Mirror(oracle, a)
+    ref = mir.ipm_map()
+    assert_fp64_map(a.magnifications, ref, "C2-shape fp64")
+    mir32 = Mirror(oracle, b, stars=stars)
+    ref32 = mir32.ipm_map()
+    e32 = rel_err(b.magnifications, ref32, floor=1e-12)
+    say(f"[mixed C2] rel err median {np.median(e32):.2e}  p99 {np.percentile(e32, 99):.2e}  max {np.max(e32):.2e}")
+    # near-critical (1 - kappa - gamma = 0.05): the mixed path keeps FP64 anchors
+    assert np.median(e32) < 1e-3, np.median(e32)
+    a.close()
+    b.close()
+
+
+def run_irs(params, **extra):
+    from irs_amd.microlensing.IRS.irs import IRS
+    kw = dict(params)
+    kw.update(extra)
+    say(f"[gpu] IRS run: {extra}")
+    m = IRS(**kw)
+    m.run()
+    return m
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_irs_binning_is_bit_exact(oracle, c1_stars, precision):
+    m = run_irs(from_starfile(C1_SMALL, c1_stars[0]), precision=precision, expansion_order=16, num_rays_y=9)
+    h1, h2 = m.half_length_y1, m.half_length_y2
+    n1, n2 = m.num_pixels_y1, m.num_pixels_y2
+    rows = np.linspace(0, m.num_ray_threads[1] - 1, 12).astype(int)
+    counts = np.zeros(n1 * n2, dtype=np.int64)
+    for r in rows:
+        w, gpu_bin = m.ray_positions(r)
+        host = np.array([oracle.irs_bin(x.real - m.center_y1, x.imag - m.center_y2, h1, h2, n1, n2)
+                         if np.isfinite(x) else -2 for x in w])
+        np.testing.assert_array_equal(gpu_bin, host)
+        ok = gpu_bin >= 0
+        np.add.at(counts, gpu_bin[ok], 1)
+    # the map kernel bins those rows exactly as the positions kernel does
+    m.shoot(int(rows[0]), int(rows[0]) + 1)
+    m.finish()
+    w, gb = m.ray_positions(int(rows[0]))
+    single = np.zeros(n1 * n2, dtype=np.int64)
+    np.add.at(single, gb[gb >= 0], 1)
+    np.testing.assert_array_equal(m.num_rays.ravel(), single)
+    m.close()
+
+
+def test_irs_map_matches_oracle(oracle, c1_stars):
+    m = run_irs(from_starfile(C1_SMALL, c1_stars[0]), precision=0, expansion_order=16, num_rays_y=9,
+                light_loss=0.001)
+    mir = Mirror(oracle, m, stars=c1_stars[1])
+    ref, got = mir.irs_map()
+    assert m.num_rays_received == got
+    assert m.num_rays.sum() == got
+    # FP64 positions differ from the oracle's by rounding only; a ray can only
+    # change pixel when it lands within ~1e-12 of a pixel edge
+    diff = np.abs(m.num_rays.astype(np.int64) - ref).sum()
+    assert diff <= 2, diff
+    m.close()
+
+
+def test_no_star_field_is_uniform(tmp_path):
+    """One negligible star far outside the field: the lens map is linear, so
+    every interior pixel has mu = 1 / ((1 - kappa)^2 - gamma^2)."""
+    from irs_amd.microlensing.IPM.ipm import IPM
+    f = tmp_path / "one_star.bin"
+    with open(f, "wb") as fh:
+        np.array([1, 0], dtype=np.int32).tofile(fh)
+        np.array([1.0e4, 0.0, 1.0], dtype=np.float64).tofile(fh)   # corner (R, 0), theta
+        np.array([1.0e4, 0.0, 1.0e-12], dtype=np.float64).tofile(fh)
+    for is_double in (True, False):
+        say(f"[gpu] no-star KAT is_double={is_double}")
+        m = IPM(kappa_tot=0.3, shear=0.2, starfile=str(f), half_length_y1=5, half_length_y2=5, num_pixels_y1=50,
+                num_pixels_y2=50, num_rays_y=9, expansion_order=3, is_double=is_double, write_stars=False,
+                write_maps=False, write_histograms=False)
+        m.run()
+        mu = 1 / ((1 - 0.3) ** 2 - 0.2 ** 2)
+        inner = m.magnifications[2:-2, 2:-2]
+        tol = 1e-9 if is_double else 2e-5
+        assert np.max(np.abs(inner / mu - 1)) < tol, np.max(np.abs(inner / mu - 1))
+        m.close()
