@@ -85,7 +85,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the map would be sharded by "
-              f"lattice rows with one reduction of the maps, see DESIGN.md)", file=sys.stderr)
+              f"lattice rows with one sum-reduction of the maps)", file=sys.stderr)
         sys.exit(2)
 
     from irs_amd.microlensing.IPM.ipm import IPM

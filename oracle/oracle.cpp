@@ -10,7 +10,7 @@ Every function cites the reference file:line it restates. The reference is
 CUDA-only and cannot be built here (no nvcc), so the restatement is pinned by
 closed forms, brute force and published known-answer vectors in tests/.
 
-Deliberate differences from the reference, all documented in DESIGN.md:
+Deliberate differences from the reference:
   * binomial coefficients are exact (uint64, rounded once to double) instead
     of int32, which wraps for rows >= 34 (binomial_coefficients.cuh:11-44);
   * star fields come from Philox4x32-10 (Salmon et al., SC'11) instead of
