@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--ref-rows", type=int, default=4096, help="rows for the reference-faithful FP64 kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-irs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the map would be sharded by "
@@ -182,6 +183,18 @@ def main():
     else:
         result["cpu_baseline"] = None
     m.close()
+    if not args.no_irs:
+        # IRS on the same configuration with the IRS light loss (irs.cuh:51): rays/s of the shooting kernel
+        from irs_amd.microlensing.IRS.irs import IRS
+        r = IRS(**dict(C2, light_loss=0.01), expansion_order=P_ORDER)
+        r.prepare()
+        r.build()
+        r.shoot()
+        r.shoot()
+        result["irs"] = {"rays": r.cells_shot, "num_stars": r.num_stars, "shoot_s": r.t_shoot_rays,
+                         "rays_per_s": r.cells_shot / r.t_shoot_rays}
+        log(f"[bench] IRS: {r.cells_shot} rays in {r.t_shoot_rays:.3f} s")
+        r.close()
     print(json.dumps(result), flush=True)
 
 

@@ -109,6 +109,13 @@ void set_expansion_order(IPMHandle* self, int val);   /* 0: automatic order (ipm
 int get_expansion_order(IPMHandle* self);
 void set_precision(IPMHandle* self, int val);         /* 0: FP64 throughout; 1: mixed FP32/FP64 (float library default) */
 int get_precision(IPMHandle* self);
+/* M2L of the FMM: 0 FP64 recurrences (default); 1 batched GEMMs on MFMA, FP32
+   for p <= 20 and FP64 above; 2 FP32 MFMA; 3 FP64 MFMA (fmm.cuh:281-374) */
+void set_m2l_mode(IPMHandle* self, int val);
+int get_m2l_mode(IPMHandle* self);
+double get_t_m2l(IPMHandle* self);          /* seconds in the M2L kernels of the last build (MFMA modes) */
+long long get_m2l_pairs(IPMHandle* self);   /* M2L (target, source) pairs of the last build (MFMA modes) */
+int get_m2l_fp32(IPMHandle* self);          /* 1 if the last build's M2L ran on FP32 MFMA */
 void set_kernel(IPMHandle* self, int val);            /* 0: fast shared-corner kernel; 1: reference-faithful FP64 kernel */
 int get_kernel(IPMHandle* self);
 int get_tree_levels(IPMHandle* self);

@@ -100,6 +100,13 @@ void set_expansion_order(IRSHandle* self, int val);
 int get_expansion_order(IRSHandle* self);
 void set_precision(IRSHandle* self, int val);
 int get_precision(IRSHandle* self);
+/* M2L of the FMM: 0 FP64 recurrences (default); 1 batched GEMMs on MFMA, FP32
+   for p <= 20 and FP64 above; 2 FP32 MFMA; 3 FP64 MFMA (fmm.cuh:281-374) */
+void set_m2l_mode(IRSHandle* self, int val);
+int get_m2l_mode(IRSHandle* self);
+double get_t_m2l(IRSHandle* self);          /* seconds in the M2L kernels of the last build (MFMA modes) */
+long long get_m2l_pairs(IRSHandle* self);   /* M2L (target, source) pairs of the last build (MFMA modes) */
+int get_m2l_fp32(IRSHandle* self);          /* 1 if the last build's M2L ran on FP32 MFMA */
 int get_tree_levels(IRSHandle* self);
 irs_dtype get_alpha_error(IRSHandle* self);
 int get_num_ray_threads_1(IRSHandle* self);

@@ -58,6 +58,7 @@
 	MLENS_FIELD(write_histograms, int, write_histograms)    \
 	MLENS_STRING_FIELD(outfile_prefix, outfile_prefix)      \
 	MLENS_FIELD(precision, int, precision)                  \
+	MLENS_FIELD(m2l_mode, int, m2l_mode)                    \
 	void set_expansion_order(HANDLE* self, int val) { self->m.expansion_order_override = val; } \
 	int get_expansion_order(HANDLE* self)                   \
 	{                                                       \
@@ -66,6 +67,9 @@
 	}
 
 #define MLENS_COMMON_ENTRIES                                                                                   \
+	double get_t_m2l(HANDLE* self) { return self->m.get_t_m2l(); }                                           \
+	long long get_m2l_pairs(HANDLE* self) { return self->m.get_m2l_pairs(); }                                \
+	int get_m2l_fp32(HANDLE* self) { return self->m.get_m2l_fp32(); }                                        \
 	int get_num_stars(HANDLE* self) { return self->m.get_num_stars(); }                                      \
 	DTYPE get_corner_x1(HANDLE* self) { return self->m.get_corner().re; }                                    \
 	DTYPE get_corner_x2(HANDLE* self) { return self->m.get_corner().im; }                                    \

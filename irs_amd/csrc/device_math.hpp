@@ -268,7 +268,9 @@ template <typename R>
 __device__ __forceinline__ R tri_pixel_area(const Tri<R>& t, R X, R Y)
 {
 	R a = 0;
-	R lo = R(1e30), hi = R(-1e30);
+	// y-range of the triangle on the line x = 1, clipped to [0, 1] below, so
+	// an empty range can start as [1, 0]
+	R lo = R(1), hi = R(0);
 #pragma unroll
 	for (int k = 0; k < 3; k++)
 	{

@@ -29,6 +29,15 @@ enum Precision
 	PREC_MIXED = 1,   // FP64 anchors, FP32 leaf-relative near field / Horner / pixel-local polygons
 };
 
+// How the M2L translations of the FMM are evaluated
+enum M2LMode
+{
+	M2L_DIRECT = 0,     // FP64 recurrences per (node, order) thread (fmm.cuh:281-374 order of operations)
+	M2L_MFMA = 1,       // batched GEMMs on MFMA: FP32 for p <= 20, FP64 above
+	M2L_MFMA_F32 = 2,   // batched GEMMs on FP32 MFMA at any p
+	M2L_MFMA_F64 = 3,   // batched GEMMs on FP64 MFMA at any p
+};
+
 enum ShootKernel
 {
 	KERNEL_FAST = 0,        // shared-corner lattice tiles, wave-uniform leaf tables
@@ -88,6 +97,11 @@ public:
 	/* FMM (P2M, M2M, L2L, M2L, P2L) and the leaf tables for shooting */
 	void fmm(int p, const LensParams& lens);
 	int expansion_order() const { return p_; }
+	void set_m2l_mode(int mode) { m2l_mode_ = mode; }
+	int m2l_mode() const { return m2l_mode_; }
+	double m2l_ms() const { return m2l_ms_; }              // GPU time of the last fmm()'s M2L kernels
+	long long m2l_pairs() const { return m2l_pairs_; }     // M2L (target, source) pairs of the last fmm()
+	bool m2l_used_f32() const { return m2l_f32_; }
 	void export_coeffs(int which, double* out) const;   // 0 = multipole, 1 = local
 	double mean_near_stars(const GridParams& g) const;  // lattice-weighted mean near-field count
 
@@ -116,6 +130,8 @@ public:
 private:
 	void ensure_node_capacity(int n);
 	void build_leaf_tables(const LensParams& lens);
+	void m2l_prepare(int p, bool f32);
+	void m2l_level(int lev, int p, bool f32, hipEvent_t e0, hipEvent_t e1);
 
 	hipStream_t stream_ = nullptr;
 	hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -135,6 +151,22 @@ private:
 	int p_ = 0;
 	DeviceBuffer<double> binom_;
 	DeviceBuffer<double> mult_, local_;   // interleaved re, im
+
+	// MFMA M2L: the 40 offset matrices (real-stacked, transposed, padded),
+	// per-level source tables and real-stacked multipoles
+	int m2l_mode_ = M2L_DIRECT;
+	double m2l_ms_ = 0;
+	long long m2l_pairs_ = 0;
+	bool m2l_f32_ = false;
+	int m2l_ready_p_ = -1;
+	bool m2l_ready_f32_ = false;
+	DeviceBuffer<float> m2l_af_;
+	DeviceBuffer<double> m2l_ad_;
+	DeviceBuffer<int> m2l_src_;
+	DeviceBuffer<float> mstack_f_;
+	DeviceBuffer<double> mstack_d_;
+	DeviceBuffer<unsigned long long> m2l_count_;
+	std::vector<hipEvent_t> m2l_ev_;
 
 	int num_leaves_ = 0;
 	DeviceBuffer<int> leaf_of_node_;

@@ -35,6 +35,7 @@ template struct DeviceBuffer<float>;
 template struct DeviceBuffer<int>;
 template struct DeviceBuffer<char>;
 template struct DeviceBuffer<long long>;
+template struct DeviceBuffer<unsigned long long>;
 
 static inline unsigned blocks_for(long long n, int threads) { return unsigned((n + threads - 1) / threads); }
 
@@ -50,6 +51,7 @@ Engine::~Engine()
 	if (stream_) (void)hipStreamSynchronize(stream_);
 	if (ev0_) (void)hipEventDestroy(ev0_);
 	if (ev1_) (void)hipEventDestroy(ev1_);
+	for (hipEvent_t e : m2l_ev_) (void)hipEventDestroy(e);
 	if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -688,7 +690,7 @@ __global__ void k_m2m(TreeView t, int l0, int l1, int p, const double* B, double
 
 // L2L (overwrite), M2L and P2L for one level (fmm.cuh:204-464)
 __global__ void k_local(TreeView t, int l0, int l1, int p, const double* B, const double* mult, double* local,
-	const double* sx, const double* sy, const double* sm)
+	const double* sx, const double* sy, const double* sm, int do_m2l)
 {
 	long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	int i = l0 + int(g / (p + 1));
@@ -709,7 +711,7 @@ __global__ void k_local(TreeView t, int l0, int l1, int p, const double* B, cons
 		r = dev::cdiv(r, dev::cpow(dz, l));
 		acc = dev::scale(r, ldexp(1.0, -l));
 	}
-	for (int j = 0; j < t.nm2l[i]; j++)
+	for (int j = 0; j < (do_m2l ? t.nm2l[i] : 0); j++)
 	{
 		int f = t.m2l[27 * i + j];
 		const long long fb = (long long)f * (p + 1);
@@ -874,16 +876,48 @@ void Engine::fmm(int p, const LensParams& lens)
 		k_m2m<<<blocks_for(work, 256), 256, 0, stream_>>>(t, l0, l1, p, binom_.ptr, mult_.ptr);
 		MLENS_HIP_CHECK(hipGetLastError());
 	}
+	// M2L: FP64 recurrences inside k_local, or batched GEMMs on MFMA after it
+	const bool mfma = m2l_mode_ != M2L_DIRECT;
+	const bool f32 = m2l_mode_ == M2L_MFMA_F32 || (m2l_mode_ == M2L_MFMA && p <= 20);
+	if (mfma)
+	{
+		m2l_prepare(p, f32);
+		if (m2l_count_.count < 2) m2l_count_.alloc(2);
+		MLENS_HIP_CHECK(hipMemsetAsync(m2l_count_.ptr, 0, 2 * sizeof(unsigned long long), stream_));
+		while (m2l_ev_.size() < size_t(2 * (levels_ + 1)))
+		{
+			hipEvent_t e;
+			MLENS_HIP_CHECK(hipEventCreate(&e));
+			m2l_ev_.push_back(e);
+		}
+	}
 	for (int lev = 2; lev <= levels_; lev++)
 	{
 		int l0 = level_start_[lev], l1 = level_start_[lev + 1];
 		long long work = (long long)(l1 - l0) * (p + 1);
 		k_local<<<blocks_for(work, 256), 256, 0, stream_>>>(t, l0, l1, p, binom_.ptr, mult_.ptr, local_.ptr, sx_.ptr,
-			sy_.ptr, sm_.ptr);
+			sy_.ptr, sm_.ptr, mfma ? 0 : 1);
 		MLENS_HIP_CHECK(hipGetLastError());
+		if (mfma) m2l_level(lev, p, f32, m2l_ev_[2 * lev], m2l_ev_[2 * lev + 1]);
 	}
 	build_leaf_tables(lens);
 	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
+	m2l_ms_ = 0;
+	m2l_pairs_ = 0;
+	m2l_f32_ = mfma && f32;
+	if (mfma)
+	{
+		for (int lev = 2; lev <= levels_; lev++)
+		{
+			float ms = 0;
+			MLENS_HIP_CHECK(hipEventElapsedTime(&ms, m2l_ev_[2 * lev], m2l_ev_[2 * lev + 1]));
+			m2l_ms_ += ms;
+		}
+		unsigned long long h[2];
+		MLENS_HIP_CHECK(hipMemcpy(h, m2l_count_.ptr, sizeof(h), hipMemcpyDeviceToHost));
+		m2l_pairs_ = (long long)h[0];
+		if (h[1] & 1) throw HipError("an M2L source is not at one of the 40 offsets of its target's level", __FILE__, __LINE__);
+	}
 }
 
 void Engine::build_leaf_tables(const LensParams& lens)

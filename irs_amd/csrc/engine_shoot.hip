@@ -438,48 +438,38 @@ __device__ __forceinline__ void acc_flush(const ShootArgs& a, PixelAcc<MapT>& c)
 	c.v0 = c.v1 = 0;
 }
 
-// general triangle: every pixel of its bounding box, areas by Green's theorem (FP64)
-template <typename MapT, bool PARITY>
-__device__ void raster_general(const ShootArgs& a, PixelAcc<MapT>& acc, int map_bit, double p0x, double p0y,
-	double p1x, double p1y, double p2x, double p2y, int X0, int X1, int Y0, int Y1)
+/*
+General triangles (crossed by more than one pixel line) are rasterised by the
+whole wave. A lane that meets one writes it, set up for per-pixel areas, into
+its wave's queue in LDS; once the queue holds more than IPM_QFLUSH triangles
+the wave walks the concatenated bounding boxes of all queued triangles, one
+pixel per lane. A lane holding the 100-pixel image of a demagnified region
+near a star then no longer stalls the other 63 (those images are a tenth of
+the general triangles and four fifths of their pixels).
+*/
+constexpr int IPM_QCAP = 80;                 // entries per wave
+constexpr int IPM_QFLUSH = IPM_QCAP - 64;    // an enqueue adds at most 64
+constexpr int IPM_QMAX_PIX = 1 << 20;        // larger boxes are walked as needles
+
+struct TriQueue
 {
-	using R = double;
-	dev::Tri<R> t;
-	t.x[0] = p0x - X0;
-	t.y[0] = p0y - Y0;
-	t.x[1] = p1x - X0;
-	t.y[1] = p1y - Y0;
-	t.x[2] = p2x - X0;
-	t.y[2] = p2y - Y0;
-	R area2 = (t.x[1] - t.x[0]) * (t.y[2] - t.y[0]) - (t.x[2] - t.x[0]) * (t.y[1] - t.y[0]);
-	if (area2 == 0) return;
-	if (area2 < 0)
-	{
-		R sx = t.x[1], sy = t.y[1];
-		t.x[1] = t.x[2];
-		t.y[1] = t.y[2];
-		t.x[2] = sx;
-		t.y[2] = sy;
-		area2 = -area2;
-	}
-	dev::tri_setup<R>(t);
-	const R scale = a.factor * dev::recip<R>(area2 * R(0.5));
-	for (int X = X0; X <= X1; X++)
-	{
-		R ylo, yhi;
-		dev::tri_strip_yrange<R>(t, R(X - X0), R(X - X0 + 1), ylo, yhi);
-		if (!(ylo <= yhi)) continue;
-		ylo = ylo > R(-1) ? ylo : R(-1);
-		yhi = yhi < R(Y1 - Y0 + 1) ? yhi : R(Y1 - Y0 + 1);
-		int Ya = max(Y0, Y0 + int(floor(ylo)));
-		int Yb = min(Y1, Y0 + int(floor(yhi)));
-		for (int Y = Ya; Y <= Yb; Y++)
-		{
-			R ar = dev::tri_pixel_area<R>(t, R(X - X0), R(Y - Y0));
-			if (ar > 0) acc_add<MapT, PARITY>(a, acc, ((long long)a.n1 * Y + X) * 2 + map_bit, MapT(scale * ar));
-		}
-	}
-}
+	double x[3][IPM_QCAP], y[3][IPM_QCAP];       // ccw vertices relative to (X0, Y0)
+	double idx[3][IPM_QCAP], idy[3][IPM_QCAP];   // reciprocal edge components
+	double scale[IPM_QCAP];                      // factor / triangle area
+	int X0[IPM_QCAP], Y0[IPM_QCAP];
+	int w[IPM_QCAP];                             // box width in pixels
+	int hkey[IPM_QCAP];                          // box height | map bit << 29 | needle << 30
+	int end[IPM_QCAP];                           // box pixels, then running total
+	float rw[IPM_QCAP];                          // 1 / w
+};
+
+// a general triangle handed from the per-lane path to the wave queue
+struct GenTri
+{
+	double x[3], y[3];   // vertices relative to (X0, Y0), in cell order
+	int X0, Y0, w, h, bit;
+	bool needle;         // box far larger than the triangle: walked by column strips
+};
 
 /*
 One triangle of a cell (polygon.cuh:349-467 semantics: its image-plane area
@@ -489,19 +479,20 @@ Fast cases: inside one pixel (everything to that pixel), or cut by a single
 pixel line: the vertex alone on one side of the line x = X cuts off a triangle
 with area fraction t_b t_c, where t_b, t_c are the fractions of its two
 edges on its own side (similar triangles); no area is ever differenced, so
-slivers stay exact.
+slivers stay exact. Returns true when the triangle is left in g for the
+wave's general rasteriser.
 */
 template <int PREC, bool PARITY, typename MapT>
-__device__ __forceinline__ void ipm_triangle(const ShootArgs& a, PixelAcc<MapT>& acc, double p0x, double p0y, double p1x,
-	double p1y, double p2x, double p2y)
+__device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>& acc, double p0x, double p0y, double p1x,
+	double p1y, double p2x, double p2y, GenTri& g)
 {
-	if (!(isfinite(p0x) && isfinite(p0y) && isfinite(p1x) && isfinite(p1y) && isfinite(p2x) && isfinite(p2y))) return;
+	if (!(isfinite(p0x) && isfinite(p0y) && isfinite(p1x) && isfinite(p1y) && isfinite(p2x) && isfinite(p2y))) return false;
 	// polygon.cuh:116-144 (difference form) and ipm_functions.cuh:176
 	double A = ((p1y - p0y) * p0x - (p1x - p0x) * p0y + (p2y - p1y) * p1x - (p2x - p1x) * p1y + (p0y - p2y) * p2x - (p0x - p2x) * p2y) / 2;
-	if (!(fabs(A) < 10000 * a.factor) || A == 0) return;
+	if (!(fabs(A) < 10000 * a.factor) || A == 0) return false;
 	const double minx = fmin(p0x, fmin(p1x, p2x)), maxx = fmax(p0x, fmax(p1x, p2x));
 	const double miny = fmin(p0y, fmin(p1y, p2y)), maxy = fmax(p0y, fmax(p1y, p2y));
-	if (minx >= a.n1 || maxx <= 0 || miny >= a.n2 || maxy <= 0) return;
+	if (minx >= a.n1 || maxx <= 0 || miny >= a.n2 || maxy <= 0) return false;
 	const int map_bit = PARITY ? (A < 0 ? 0 : 1) : 0;
 	const int X0 = int(floor(fmax(minx, 0.0)));
 	const int Y0 = int(floor(fmax(miny, 0.0)));
@@ -512,7 +503,7 @@ __device__ __forceinline__ void ipm_triangle(const ShootArgs& a, PixelAcc<MapT>&
 	if (inside && X0 == X1 && Y0 == Y1)
 	{
 		acc_add<MapT, PARITY>(a, acc, ((long long)a.n1 * Y0 + X0) * 2 + map_bit, factor);
-		return;
+		return false;
 	}
 	const bool xcut = inside && X1 == X0 + 1 && Y0 == Y1;
 	const bool ycut = inside && Y1 == Y0 + 1 && X0 == X1;
@@ -544,17 +535,239 @@ __device__ __forceinline__ void ipm_triangle(const ShootArgs& a, PixelAcc<MapT>&
 			ub = u0;
 			uc = u1;
 		}
-		// fractions of the edges from the lone vertex to the line
-		const double tb = ua / (ua - ub), tc = ua / (ua - uc);
-		const MapT lone = MapT(double(a.factor) * (tb * tc));
+		// fractions t_b = ua / (ua - ub), t_c = ua / (ua - uc) of the edges from
+		// the lone vertex to the line; their product in one division
+		const MapT lone = MapT(double(a.factor) * ((ua * ua) / ((ua - ub) * (ua - uc))));
 		long long klow = (long long)a.n1 * Y0 + X0;
 		long long khigh = xcut ? klow + 1 : klow + a.n1;
 		long long klone = lone_low ? klow : khigh, kother = lone_low ? khigh : klow;
 		acc_add<MapT, PARITY>(a, acc, klone * 2 + map_bit, lone);
 		acc_add<MapT, PARITY>(a, acc, kother * 2 + map_bit, factor - lone);
-		return;
+		return false;
 	}
-	raster_general<MapT, PARITY>(a, acc, map_bit, p0x, p0y, p1x, p1y, p2x, p2y, X0, X1, Y0, Y1);
+	// general: FP64 vertices relative to the box origin; the wave's queue
+	// orients them and sets up the edges
+	const int w = X1 - X0 + 1, h = Y1 - Y0 + 1;
+	g.x[0] = p0x - X0;
+	g.y[0] = p0y - Y0;
+	g.x[1] = p1x - X0;
+	g.y[1] = p1y - Y0;
+	g.x[2] = p2x - X0;
+	g.y[2] = p2y - Y0;
+	const double area2 = fabs((g.x[1] - g.x[0]) * (g.y[2] - g.y[0]) - (g.x[2] - g.x[0]) * (g.y[1] - g.y[0]));
+	if (area2 == 0) return false;
+	// needles: a box much larger than the triangle is walked by column strips
+	const long long npix = (long long)w * h;
+	g.needle = npix > IPM_QMAX_PIX || double(npix) > 2.0 * area2 + 4.0 * double(w + h) + 64.0;
+	g.X0 = X0;
+	g.Y0 = Y0;
+	g.w = w;
+	g.h = h;
+	g.bit = map_bit;
+	return true;
+}
+
+// orders this wave's LDS accesses: the compiler keeps them in program order
+// and the wave waits for its outstanding LDS operations
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// rasterise every queued triangle: pixel k of the concatenated boxes goes to lane k mod 64
+template <typename MapT, bool PARITY>
+__device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q, int qn)
+{
+	const double factor = a.factor;
+	const int lane = threadIdx.x & 63;
+	wave_lds_sync();
+	// set-up: counter-clockwise order, reciprocal edge components, factor / area
+	for (int base = 0; base < qn; base += 64)
+	{
+		int e = base + lane;
+		asm volatile("" : "+v"(e));
+		if (e < qn)
+		{
+			dev::Tri<double> t;
+#pragma unroll
+			for (int i = 0; i < 3; i++)
+			{
+				t.x[i] = q.x[i][e];
+				t.y[i] = q.y[i][e];
+			}
+			double area2 = (t.x[1] - t.x[0]) * (t.y[2] - t.y[0]) - (t.x[2] - t.x[0]) * (t.y[1] - t.y[0]);
+			if (area2 < 0)
+			{
+				const double sx = t.x[1], sy = t.y[1];
+				t.x[1] = t.x[2];
+				t.y[1] = t.y[2];
+				t.x[2] = sx;
+				t.y[2] = sy;
+				area2 = -area2;
+				q.x[1][e] = t.x[1];
+				q.y[1][e] = t.y[1];
+				q.x[2][e] = t.x[2];
+				q.y[2][e] = t.y[2];
+			}
+			dev::tri_setup<double>(t);
+#pragma unroll
+			for (int i = 0; i < 3; i++)
+			{
+				q.idx[i][e] = t.idx[i];
+				q.idy[i][e] = t.idy[i];
+			}
+			q.scale[e] = factor * dev::recip<double>(area2 * 0.5);
+		}
+	}
+	int total = 0;
+	for (int base = 0; base < qn; base += 64)
+	{
+		int e = base + lane;
+		asm volatile("" : "+v"(e));
+		int c = e < qn ? q.end[e] : 0;
+		// source lanes from an opaque lane index, so that their addresses are
+		// formed here and not held through the shooting loop
+		int ln = lane;
+		asm volatile("" : "+v"(ln));
+#pragma unroll
+		for (int d = 1; d < 64; d <<= 1)
+		{
+			const int u = __shfl(c, ln >= d ? ln - d : ln);
+			if (ln >= d) c += u;
+		}
+		if (e < qn) q.end[e] = total + c;
+		total += __shfl(c, 63);
+	}
+	wave_lds_sync();
+	for (int k0 = 0; k0 < total; k0 += 64)
+	{
+		const int k = k0 + lane;
+		if (k < total)
+		{
+			// the entry whose running total first exceeds k
+			int lo = 0, hi = qn - 1;
+			while (lo < hi)
+			{
+				const int mid = (lo + hi) >> 1;
+				if (q.end[mid] > k)
+					hi = mid;
+				else
+					lo = mid + 1;
+			}
+			const int e = lo;
+			const int local = k - (e > 0 ? q.end[e - 1] : 0);
+			const int w = q.w[e];
+			// local < 2^20: the float quotient is off by at most one
+			int yy = int(float(local) * q.rw[e]);
+			int xx = local - yy * w;
+			if (xx >= w)
+			{
+				yy++;
+				xx -= w;
+			}
+			else if (xx < 0)
+			{
+				yy--;
+				xx += w;
+			}
+			dev::Tri<double> t;
+#pragma unroll
+			for (int i = 0; i < 3; i++)
+			{
+				t.x[i] = q.x[i][e];
+				t.y[i] = q.y[i][e];
+				t.idx[i] = q.idx[i][e];
+				t.idy[i] = q.idy[i][e];
+			}
+#pragma unroll
+			for (int i = 0; i < 3; i++)
+			{
+				const int n = (i + 1) % 3;
+				t.dx[i] = t.x[n] - t.x[i];
+				t.dy[i] = t.y[n] - t.y[i];
+			}
+			const double ar = dev::tri_pixel_area<double>(t, double(xx), double(yy));
+			if (ar > 0)
+			{
+				const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx);
+				acc_flush_one<MapT, PARITY>(a, pix * 2 + ((q.hkey[e] >> 29) & 1), MapT(q.scale[e] * ar));
+			}
+		}
+	}
+	// needles: one column strip per lane, pixels of the strip found from the
+	// triangle's y-range inside it
+	for (int base = 0; base < qn; base += 64)
+	{
+		int e0 = base + lane;
+		asm volatile("" : "+v"(e0));
+		unsigned long long needles = __ballot(e0 < qn && ((q.hkey[e0] >> 30) & 1));
+		while (needles)
+		{
+			const int e = base + __ffsll((long long)needles) - 1;
+			needles &= needles - 1;
+			dev::Tri<double> t;
+#pragma unroll
+			for (int i = 0; i < 3; i++)
+			{
+				t.x[i] = q.x[i][e];
+				t.y[i] = q.y[i][e];
+				t.idx[i] = q.idx[i][e];
+				t.idy[i] = q.idy[i][e];
+			}
+#pragma unroll
+			for (int i = 0; i < 3; i++)
+			{
+				const int n = (i + 1) % 3;
+				t.dx[i] = t.x[n] - t.x[i];
+				t.dy[i] = t.y[n] - t.y[i];
+			}
+			const int w = q.w[e], h = q.hkey[e] & ((1 << 29) - 1), X0 = q.X0[e], Y0 = q.Y0[e], bit = (q.hkey[e] >> 29) & 1;
+			const double scale = q.scale[e];
+			for (int c = lane; c < w; c += 64)
+			{
+				double ylo, yhi;
+				dev::tri_strip_yrange<double>(t, double(c), double(c + 1), ylo, yhi);
+				if (!(ylo <= yhi)) continue;
+				ylo = ylo > -1.0 ? ylo : -1.0;
+				yhi = yhi < double(h) ? yhi : double(h);
+				const int ya = max(0, int(floor(ylo))), yb = min(h - 1, int(floor(yhi)));
+				for (int yy = ya; yy <= yb; yy++)
+				{
+					const double ar = dev::tri_pixel_area<double>(t, double(c), double(yy));
+					if (ar > 0)
+					{
+						const long long pix = (long long)a.n1 * (Y0 + yy) + (X0 + c);
+						acc_flush_one<MapT, PARITY>(a, pix * 2 + bit, MapT(scale * ar));
+					}
+				}
+			}
+		}
+	}
+	wave_lds_sync();
+}
+
+// wave-wide: lanes with gen set append g to the queue
+__device__ __forceinline__ void tri_queue_push(TriQueue& q, int& qn, bool gen, const GenTri& g)
+{
+	const unsigned long long m = __ballot(gen);
+	if (m == 0) return;
+	if (gen)
+	{
+		int pos = qn + int(__builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u)));
+		// keep the index opaque: otherwise the lane part of every field's
+		// address is hoisted out of the shooting loop, one register each
+		asm volatile("" : "+v"(pos));
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+		{
+			q.x[i][pos] = g.x[i];
+			q.y[i][pos] = g.y[i];
+		}
+		q.X0[pos] = g.X0;
+		q.Y0[pos] = g.Y0;
+		q.w[pos] = g.w;
+		q.hkey[pos] = g.h | (g.bit << 29) | (g.needle ? 1 << 30 : 0);
+		q.end[pos] = g.needle ? 0 : g.w * g.h;
+		q.rw[pos] = 1.0f / float(g.w);
+	}
+	qn += __popcll(m);
 }
 
 constexpr int IPM_WAVES = 4;
@@ -581,6 +794,9 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long l
 	LeafCache cache;
 	using MapT = typename std::conditional<PREC == PREC_MIXED, float, double>::type;
 	PixelAcc<MapT> acc;
+	__shared__ TriQueue queues[IPM_WAVES];
+	TriQueue& q = queues[wave];
+	int qn = 0;
 
 	auto corner = [&](long long J, double& px, double& py) {
 		double zy = a.y0 + a.stepy * double(J);
@@ -598,11 +814,23 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long l
 		double tx, ty;
 		corner(J, tx, ty);
 		double trx = __shfl_down(tx, 1), try_ = __shfl_down(ty, 1);
-		if (cell_ok)
+		// cell (I, J-1): y0 = upper right, y1 = upper left, y2 = lower left, y3 = lower right;
+		// triangles (y0, y1, y2) and (y2, y3, y0). One call site each for the
+		// triangle, the queue and its flush keeps the register footprint small.
+#pragma unroll 1
+		for (int t = 0; t < 2; t++)
 		{
-			// cell (I, J-1): y0 = upper right, y1 = upper left, y2 = lower left, y3 = lower right
-			ipm_triangle<PREC, PARITY, MapT>(a, acc, trx, try_, tx, ty, bx, by);
-			ipm_triangle<PREC, PARITY, MapT>(a, acc, bx, by, brx, bry, trx, try_);
+			const double ax = t ? bx : trx, ay = t ? by : try_;
+			const double bxx = t ? brx : tx, byy = t ? bry : ty;
+			const double cx = t ? trx : bx, cy = t ? try_ : by;
+			GenTri g;
+			const bool gen = cell_ok && ipm_triangle<PREC, PARITY, MapT>(a, acc, ax, ay, bxx, byy, cx, cy, g);
+			tri_queue_push(q, qn, gen, g);
+			if (qn > IPM_QFLUSH || (qn > 0 && t == 1 && J == J1))
+			{
+				tri_queue_flush<MapT, PARITY>(a, q, qn);
+				qn = 0;
+			}
 		}
 		bx = tx;
 		by = ty;
@@ -881,22 +1109,54 @@ IRS (irs_functions.cuh:111-212)
 --------------------------------------------------------------------------- */
 // irs_functions.cuh:152-184, array_functions.cuh:70-75, bit-for-bit the
 // reference's operation order; -1 when the ray misses the map
-__device__ __forceinline__ long long irs_bin(const ShootArgs& a, double wx, double wy)
+__device__ __forceinline__ bool irs_pixel(const ShootArgs& a, double wx, double wy, int& px, int& py)
 {
 #pragma clang fp contract(off)
 	double w1 = wx - a.cyx;
 	double w2 = wy - a.cyy;
-	if (w1 < -a.h1 || w1 > a.h1 || w2 < -a.h2 || w2 > a.h2) return -1;
-	int px = int((w1 + a.h1) * a.n1 / (2 * a.h1));
-	int py = int((w2 + a.h2) * a.n2 / (2 * a.h2));
+	if (w1 < -a.h1 || w1 > a.h1 || w2 < -a.h2 || w2 > a.h2) return false;
+	px = int((w1 + a.h1) * a.n1 / (2 * a.h1));
+	py = int((w2 + a.h2) * a.n2 / (2 * a.h2));
 	if (px == a.n1) px--;
 	if (py == a.n2) py--;
 	py = a.n2 - 1 - py;
+	return true;
+}
+
+__device__ __forceinline__ long long irs_bin(const ShootArgs& a, double wx, double wy)
+{
+	int px, py;
+	if (!irs_pixel(a, wx, wy, px, py)) return -1;
 	return (long long)py * a.n1 + px;
 }
 
 constexpr int IRS_THREADS = 256;
 constexpr int IRS_TILE_H = 32;
+constexpr int IRS_WX = 128, IRS_WY = 64;   // LDS window of ray counts, 32 KB
+
+/*
+Ray counts go through three levels: a per-lane two-slot register cache
+(consecutive rows of a lane usually land in one pixel), a per-block LDS
+window of IRS_WX x IRS_WY pixels centred on the block's first row of rays
+(a block's rays land in a few dozen pixels away from stars), and the map in
+HBM for everything else. Only evicted register slots touch LDS, only the
+window's nonzero pixels and the rays that miss it touch HBM. Counts are
+integers, so the map is independent of the order.
+*/
+struct IrsWindow
+{
+	int* w;
+	int x0, y0;
+};
+
+__device__ __forceinline__ void irs_count(const ShootArgs& a, const IrsWindow& win, int px, int py, int v)
+{
+	const int dx = px - win.x0, dy = py - win.y0;
+	if (dx >= 0 && dx < IRS_WX && dy >= 0 && dy < IRS_WY)
+		atomicAdd(win.w + dy * IRS_WX + dx, v);
+	else
+		atomicAdd(reinterpret_cast<int*>(a.map) + ((long long)py * a.n1 + px), v);
+}
 
 template <int PREC, bool PARITY>
 __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long row0, long long row1,
@@ -907,6 +1167,8 @@ __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long
 	a.map = map;
 	a.mmin = mmin;
 	a.msad = msad;
+	__shared__ int win_counts[IRS_WX * IRS_WY];
+	__shared__ int box[4];
 	const long long i = (long long)blockIdx.x * IRS_THREADS + threadIdx.x;
 	const long long J0 = row0 + (long long)blockIdx.y * IRS_TILE_H;
 	if (J0 >= row1) return;
@@ -915,21 +1177,91 @@ __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long
 	const double zx = a.x0 + a.stepx * double(ok ? i : a.N1 - 1);
 	LeafCache cache;
 	unsigned long long got = 0;
-	// ray counts summed per lane before the atomics (see PixelAcc)
-	PixelAcc<int> tot, par;
+	for (int k = threadIdx.x; k < IRS_WX * IRS_WY; k += IRS_THREADS) win_counts[k] = 0;
+	if (threadIdx.x == 0)
+	{
+		box[0] = box[1] = 0x7fffffff;
+		box[2] = box[3] = -0x7fffffff;
+	}
+	__syncthreads();
+
+	// register cache: (x, y) of the two most recent pixels and their counts
+	int kx0 = -1, ky0 = -1, c0 = 0, kx1 = -1, ky1 = -1, c1 = 0;
+	PixelAcc<int> par;
+	IrsWindow win{win_counts, 0, 0};
 	for (long long j = J0; j < J1; j++)
 	{
 		double zy = a.y0 + a.stepy * double(j);
 		Deflection d = deflect<PREC, PARITY>(a, zx, zy, cache);
-		if (!ok || isnan(d.wx) || isnan(d.wy)) continue;
-		long long idx = irs_bin(a, d.wx, d.wy);
-		if (idx < 0) continue;
-		if (PARITY) acc_add<int, true>(a, par, idx * 2 + (d.mu >= 0 ? 0 : 1), 1);
-		acc_add<int, false>(a, tot, idx * 2, 1);
+		int px = 0, py = 0;
+		const bool hit = ok && !isnan(d.wx) && !isnan(d.wy) && irs_pixel(a, d.wx, d.wy, px, py);
+		if (j == J0)
+		{
+			// window centred on the bounding box of the block's first row
+			int mnx = hit ? px : 0x7fffffff, mny = hit ? py : 0x7fffffff;
+			int mxx = hit ? px : -0x7fffffff, mxy = hit ? py : -0x7fffffff;
+			for (int off = 32; off > 0; off >>= 1)
+			{
+				mnx = min(mnx, __shfl_xor(mnx, off));
+				mny = min(mny, __shfl_xor(mny, off));
+				mxx = max(mxx, __shfl_xor(mxx, off));
+				mxy = max(mxy, __shfl_xor(mxy, off));
+			}
+			if ((threadIdx.x & 63) == 0)
+			{
+				atomicMin(&box[0], mnx);
+				atomicMin(&box[1], mny);
+				atomicMax(&box[2], mxx);
+				atomicMax(&box[3], mxy);
+			}
+			__syncthreads();
+			if (box[0] <= box[2])
+			{
+				win.x0 = (box[0] + box[2]) / 2 - IRS_WX / 2;
+				win.y0 = (box[1] + box[3]) / 2 - IRS_WY / 2;
+			}
+			else
+			{
+				win.x0 = win.y0 = -0x3fffffff;   // no ray of the first row landed: no window
+			}
+		}
+		if (!hit) continue;
+		if (PARITY) acc_add<int, true>(a, par, ((long long)py * a.n1 + px) * 2 + (d.mu >= 0 ? 0 : 1), 1);
+		if (px == kx0 && py == ky0)
+		{
+			c0++;
+		}
+		else if (px == kx1 && py == ky1)
+		{
+			const int t = c1 + 1;
+			kx1 = kx0;
+			ky1 = ky0;
+			c1 = c0;
+			kx0 = px;
+			ky0 = py;
+			c0 = t;
+		}
+		else
+		{
+			if (c1) irs_count(a, win, kx1, ky1, c1);
+			kx1 = kx0;
+			ky1 = ky0;
+			c1 = c0;
+			kx0 = px;
+			ky0 = py;
+			c0 = 1;
+		}
 		got++;
 	}
-	acc_flush<int, false>(a, tot);
+	if (c0) irs_count(a, win, kx0, ky0, c0);
+	if (c1) irs_count(a, win, kx1, ky1, c1);
 	if (PARITY) acc_flush<int, true>(a, par);
+	__syncthreads();
+	for (int k = threadIdx.x; k < IRS_WX * IRS_WY; k += IRS_THREADS)
+	{
+		const int v = win_counts[k];
+		if (v) atomicAdd(reinterpret_cast<int*>(a.map) + ((long long)(win.y0 + k / IRS_WX) * a.n1 + (win.x0 + k % IRS_WX)), v);
+	}
 	// wave reduction, one 64-bit atomic per wave
 	for (int off = 32; off > 0; off >>= 1) got += __shfl_down(got, off);
 	if ((threadIdx.x & 63) == 0 && got) atomicAdd(received, got);

@@ -70,6 +70,7 @@ public:
 
 	/* extensions (no reference counterpart) */
 	int expansion_order_override = 0;   // 0: automatic order of ipm.cuh:848-860
+	int m2l_mode = M2L_DIRECT;          // M2LMode: FP64 recurrences or batched MFMA GEMMs
 	int kernel = KERNEL_FAST;
 	int precision = std::is_same<T, float>::value ? PREC_MIXED : PREC_FP64;
 
@@ -198,6 +199,9 @@ public:
 	long long get_num_rays_shot() const { return (long long)num_ray_threads_.re * num_ray_threads_.im; }
 	int get_tree_levels() const { return tree_levels_; }
 	int get_expansion_order() const { return expansion_order_; }
+	double get_t_m2l() const { return engine_ ? engine_->m2l_ms() / 1000.0 : 0.0; }
+	long long get_m2l_pairs() const { return engine_ ? engine_->m2l_pairs() : 0; }
+	int get_m2l_fp32() const { return engine_ && engine_->m2l_used_f32() ? 1 : 0; }
 	T get_alpha_error() const { return alpha_error_; }
 	T get_mu_ave() const { return mu_ave_; }
 	T get_root_half_length() const { return root_half_length_; }
@@ -392,6 +396,11 @@ private:
 		if (expansion_order_override < 0 || expansion_order_override > MAX_EXPANSION_ORDER)
 		{
 			std::cerr << "Error. expansion_order must be 0 (automatic) or between 1 and " << MAX_EXPANSION_ORDER << "\n";
+			return false;
+		}
+		if (m2l_mode < M2L_DIRECT || m2l_mode > M2L_MFMA_F64)
+		{
+			std::cerr << "Error. m2l_mode must be 0 (direct FP64), 1 (MFMA, FP32 for p <= 20), 2 (FP32 MFMA) or 3 (FP64 MFMA).\n";
 			return false;
 		}
 		if (kernel != KERNEL_FAST && kernel != KERNEL_REFERENCE)
@@ -655,6 +664,7 @@ private:
 			}
 		}
 		MLENS_VERBOSE("Calculating multipole and local coefficients...\n", verbose, 1);
+		engine_->set_m2l_mode(m2l_mode);
 		engine_->fmm(expansion_order_, lens());
 		MLENS_VERBOSE("Done calculating multipole and local coefficients.\n\n", verbose, 1);
 		return true;

@@ -4,7 +4,9 @@ properties, run()/save(), result attributes and plots, over this engine's
 lib_ipm.so / lib_ipm_double.so.
 
 Additions: `expansion_order` (force the FMM order), `precision` (0 = FP64,
-1 = mixed), `kernel` (0 = fast, 1 = reference-faithful FP64), the staged
+1 = mixed), `kernel` (0 = fast, 1 = reference-faithful FP64), `m2l_mode`
+(0 = FP64 recurrences, 1 = MFMA with FP32 up to p = 20 and FP64 above,
+2 = FP32 MFMA, 3 = FP64 MFMA) with `t_m2l`/`m2l_pairs`, the staged
 `prepare/build/shoot/finish` used by the benchmark, `deflect` for point
 evaluations, and `close()`, which releases the engine object (the reference
 never frees it).
@@ -28,7 +30,7 @@ class IPM(object):
                  num_rays_y: int = None, random_seed: int = None, write_stars: bool = None,
                  write_maps: bool = None, write_parities: bool = None, write_histograms: bool = None,
                  outfile_prefix: str = None, verbose: int = 0, is_double: bool = False,
-                 expansion_order: int = None, precision: int = None, kernel: int = None):
+                 expansion_order: int = None, precision: int = None, kernel: int = None, m2l_mode: int = None):
         """Arguments as in the reference (ipm.py:15-45); see the class docstring for the additions."""
         self.is_double = is_double
         self.lib = self._library(is_double)
@@ -71,6 +73,7 @@ class IPM(object):
         self.expansion_order = expansion_order
         self.precision = precision
         self.kernel = kernel
+        self.m2l_mode = m2l_mode
 
         self.magnifications = None
         self.magnifications_minima = None
@@ -405,6 +408,30 @@ class IPM(object):
             if value not in (0, 1):
                 raise ValueError("precision must be 0 (fp64) or 1 (mixed)")
             self.lib.set_precision(self.obj, int(value))
+
+    @property
+    def m2l_mode(self):
+        return self.lib.get_m2l_mode(self.obj)
+
+    @m2l_mode.setter
+    def m2l_mode(self, value):
+        if value is not None:
+            if value not in (0, 1, 2, 3):
+                raise ValueError("m2l_mode must be 0 (direct FP64), 1 (MFMA), 2 (FP32 MFMA) or 3 (FP64 MFMA)")
+            self.lib.set_m2l_mode(self.obj, int(value))
+
+    @property
+    def t_m2l(self):
+        """Seconds in the MFMA M2L kernels of the last build."""
+        return self.lib.get_t_m2l(self.obj)
+
+    @property
+    def m2l_pairs(self):
+        return self.lib.get_m2l_pairs(self.obj)
+
+    @property
+    def m2l_fp32(self):
+        return bool(self.lib.get_m2l_fp32(self.obj))
 
     @property
     def kernel(self):

@@ -16,7 +16,7 @@ _FIELDS = [
     ("half_length_y1", "d"), ("half_length_y2", "d"), ("num_pixels_y1", "i"), ("num_pixels_y2", "i"),
     ("num_rays_y", "i"), ("random_seed", "i"), ("write_stars", "i"), ("write_maps", "i"),
     ("write_parities", "i"), ("write_histograms", "i"), ("outfile_prefix", "s"),
-    ("expansion_order", "i"), ("precision", "i"),
+    ("expansion_order", "i"), ("precision", "i"), ("m2l_mode", "i"),
 ]
 
 
@@ -59,6 +59,9 @@ def load(path, dtype, irs=False):
     sig("get_num_nodes", [v], i)
     sig("get_cells_shot", [v], ll)
     sig("get_taylor_smooth", [v], i)
+    sig("get_t_m2l", [v], ctypes.c_double)
+    sig("get_m2l_pairs", [v], ll)
+    sig("get_m2l_fp32", [v], i)
     for name in _DERIVED:
         sig(name, [v], d)
     sig("run", [v, i], b)
@@ -94,6 +97,8 @@ def load(path, dtype, irs=False):
     return lib
 
 
+_M2L_SYMBOLS = ["set_m2l_mode", "get_m2l_mode", "get_t_m2l", "get_m2l_pairs", "get_m2l_fp32"]
+
 # every symbol include/lib_ipm.h declares (the first 63 are the reference's)
 IPM_SYMBOLS = (["init"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f"get_{n}" for n, _ in _FIELDS[:26]] +
                ["get_num_stars", "get_corner_x1", "get_corner_x2", "get_stars", "get_pixels", "get_pixels_minima",
@@ -103,7 +108,7 @@ IPM_SYMBOLS = (["init"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f"get_{n}" fo
                 "get_num_ray_threads_2", "get_root_half_length", "get_mean_near_stars", "get_taylor_smooth"] + _DERIVED +
                ["ipm_prepare", "ipm_build",
                 "ipm_shoot", "ipm_finish", "get_cells_shot", "ipm_deflect", "get_num_nodes", "ipm_export_tree",
-                "ipm_export_coeffs"])
+                "ipm_export_coeffs"] + _M2L_SYMBOLS)
 
 IRS_SYMBOLS = (["init", "irs_free"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f"get_{n}" for n, _ in _FIELDS[:26]] +
                ["get_num_stars", "get_corner_x1", "get_corner_x2", "get_stars", "get_pixels", "get_pixels_minima",
@@ -112,4 +117,4 @@ IRS_SYMBOLS = (["init", "irs_free"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f
                 "get_alpha_error", "get_num_ray_threads_1", "get_num_ray_threads_2", "get_root_half_length",
                 "get_mean_near_stars", "get_num_nodes", "get_cells_shot", "get_taylor_smooth"] + _DERIVED +
                ["irs_prepare", "irs_build", "irs_shoot",
-                "irs_finish", "irs_ray_positions"])
+                "irs_finish", "irs_ray_positions"] + _M2L_SYMBOLS)

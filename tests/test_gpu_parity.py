@@ -87,6 +87,32 @@ def test_fmm_coefficients_match_oracle(c1_fp64, c1_mirror):
         assert err < 1e-11, (which, err)
 
 
+@pytest.mark.parametrize("mode,tol", [(3, 1e-11), (2, 2e-5), (1, 2e-5)])
+def test_m2l_on_mfma_matches_oracle(oracle, c1_stars, c1_fp64, c1_mirror, mode, tol):
+    """M2L as batched GEMMs on the matrix cores against the oracle's FP64 M2L
+    (fmm.cuh:281-374): FP64 MFMA agrees to rounding; FP32 MFMA to single
+    precision of the products. Mode 1 at p = 16 runs in FP32."""
+    say(f"[gpu] m2l_mode {mode}")
+    m = run_ipm(from_starfile(C1_SMALL, c1_stars[0]), is_double=True, expansion_order=16, m2l_mode=mode)
+    g = m.export_coeffs("local")
+    o = c1_mirror.tree.coeffs("local")
+    scale = np.maximum(np.max(np.abs(o), axis=1, keepdims=True), 1e-300)
+    err = np.max(np.abs(g - o) / scale)
+    say(f"[gpu] m2l_mode {mode}: max local-coefficient error {err:.2e}")
+    assert err < tol
+    assert m.m2l_fp32 == (mode in (1, 2))
+    # every M2L pair of the tree found its offset matrix
+    _, info, _ = c1_fp64.export_tree()
+    assert m.m2l_pairs == int(info[info[:, 0] >= 2, 7].sum()) > 0
+    assert m.t_m2l > 0
+    # the deflection stays within the reference's FMM budget of the direct sum
+    z = random_points(m, 20000, seed=5)
+    w = m.deflect(z)
+    w_direct = oracle.deflect_direct(c1_mirror.stars, c1_mirror.lens, z)
+    assert np.max(np.abs(w - w_direct)) < m.alpha_error
+    m.close()
+
+
 def test_deflection_matches_oracle(oracle, c1_fp64, c1_mirror):
     z = random_points(c1_fp64, 20000, seed=3)
     w_gpu = c1_fp64.deflect(z)
@@ -240,3 +266,37 @@ def test_no_star_field_is_uniform(tmp_path):
         tol = 1e-9 if is_double else 2e-5
         assert np.max(np.abs(inner / mu - 1)) < tol, np.max(np.abs(inner / mu - 1))
         m.close()
+
+
+@pytest.mark.parametrize("is_double", [True, False])
+def test_saved_files_read_back(tmp_path, c1_stars, is_double):
+    """save() writes the reference's files (ipm.cuh:1122-1322): every one reads
+    back through the Util helpers to exactly what the library returned."""
+    from irs_amd.microlensing.IPM.ipm import IPM
+    from irs_amd.microlensing.Util import util
+    prefix = str(tmp_path) + "/"
+    kw = from_starfile(C1_SMALL, c1_stars[0])
+    kw.update(write_stars=True, write_maps=True, write_parities=True, write_histograms=True, outfile_prefix=prefix)
+    say(f"[gpu] save and read back, is_double={is_double}")
+    m = IPM(**kw, is_double=is_double, expansion_order=12)
+    m.run()
+    m.save()
+    dt = np.float64 if is_double else np.float32
+    assert np.array_equal(util.read_array(prefix + "ipm_magnifications.bin", dt), m.magnifications)
+    assert np.array_equal(util.read_array(prefix + "ipm_magnifications_minima.bin", dt), m.magnifications_minima)
+    assert np.array_equal(util.read_array(prefix + "ipm_magnifications_saddles.bin", dt), m.magnifications_saddles)
+    stars, rect, corner, theta = util.read_stars(prefix + "ipm_stars.bin", dtype=dt)
+    assert np.array_equal(stars, m.stars) and not rect and theta == 1.0
+    p = util.read_params(prefix + "ipm_parameter_info.txt")
+    assert p["num_stars"] == m.num_stars
+    assert p["num_pixels_y1"] == C1_SMALL["num_pixels_y1"] and p["kappa_tot"] == pytest.approx(C1_SMALL["kappa_tot"])
+    npix = C1_SMALL["num_pixels_y1"] * C1_SMALL["num_pixels_y2"]
+    for suffix in ("", "_minima", "_saddles"):
+        h = util.read_hist(prefix + f"ipm_mags_numpixels{suffix}.txt")
+        assert h[:, 1].sum() == npix
+    # the linear histogram bins mu * 1000 rounded half away from zero (ipm.cuh:1003-1010)
+    h = util.read_hist(prefix + "ipm_mags_numpixels.txt")
+    v = m.magnifications.astype(np.float64).ravel() * 1000.0 - float(h[0, 0])
+    ref = np.bincount(np.floor(v + 0.5).astype(np.int64))
+    assert np.array_equal(h[:, 1], ref[ref > 0])
+    m.close()
