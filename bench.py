@@ -38,6 +38,19 @@ PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 = unpacked FP32 (packed FP32 in
 PEAK_PACKED_FP32_TFLOPS = 157.3
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_ipm_fast_10000px_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of k_ipm_fast from the FETCH_SIZE / WRITE_SIZE
+    profile of this workload, scaled by a 1 GiB copy (see that file)."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            return float(json.load(fh)["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -73,6 +86,58 @@ def cpu_baseline(m, p, seconds):
                        f"(tree/FMM setup {t_setup:.1f} s not timed)")
 
 
+# FMM stress workload: kappa = kappa_star = 0.8, shear 0.1, no smooth matter, Salpeter 0.1-1 M_sun,
+# 20000^2 pixels over half-length 50
+C3 = dict(kappa_tot=0.8, shear=0.1, kappa_star=0.8, theta_star=1.0, mass_function="salpeter", m_solar=1.0,
+          m_lower=0.1, m_upper=1.0, light_loss=0.001, rectangular=False, approx=True, safety_scale=1.37,
+          center_y1=0.0, center_y2=0.0, half_length_y1=50.0, half_length_y2=50.0, num_pixels_y1=20000,
+          num_pixels_y2=20000, num_rays_y=1, random_seed=3, write_stars=False, write_maps=False,
+          write_histograms=False)
+PEAK_MFMA_TFLOPS = {True: 157.3, False: 78.6}   # dense FP32 / FP64 MFMA
+
+
+def m2l_stress(orders, npoints=200000):
+    """M2L on FP32 and FP64 MFMA against the FP64 recurrences on the FMM stress
+    workload: local-coefficient and deflection errors, M2L kernel time and MFMA
+    utilisation (useful FLOP = 8 (p+1)^2 per source-target pair)."""
+    from irs_amd.microlensing.IPM.ipm import IPM
+    rows = []
+    for p in orders:
+        ref = None
+        for mode in (0, 2, 3):
+            m = IPM(**C3, is_double=True, precision=0, expansion_order=p, m2l_mode=mode)
+            m.prepare()
+            m.build()
+            m.build()   # second build: warm kernels and buffers
+            coeffs = m.export_coeffs("local")
+            rng = np.random.default_rng(11)
+            hx, hy = m.lib.get_half_length_x1(m.obj), m.lib.get_half_length_x2(m.obj)
+            z = hx * rng.uniform(-1, 1, npoints) + 1j * hy * rng.uniform(-1, 1, npoints)
+            w = m.deflect(z)
+            if mode == 0:
+                ref = (coeffs, w, z)
+                log(f"[m2l] p={p}: {m.num_stars} stars, {m.tree_levels} levels")
+            else:
+                c0, w0, _ = ref
+                scale = np.maximum(np.max(np.abs(c0), axis=1, keepdims=True), 1e-300)
+                ec = (np.abs(coeffs - c0) / scale).max(axis=1)
+                # deflection error relative to the far-field (local) part it perturbs: |alpha - alpha_ref| / |w - z|
+                ew = np.abs(w - w0) / np.maximum(np.abs(w0 - z), 1e-300)
+                f32 = m.m2l_fp32
+                pairs = m.m2l_pairs
+                tflops = pairs * 8.0 * (p + 1) ** 2 / m.t_m2l / 1e12
+                row = {"p": p, "mfma": "fp32" if f32 else "fp64", "num_stars": m.num_stars, "pairs": pairs,
+                       "m2l_ms": 1e3 * m.t_m2l, "tflops": tflops, "mfma_util": tflops / PEAK_MFMA_TFLOPS[f32],
+                       "coeff_rel_err_median": float(np.median(ec)), "coeff_rel_err_max": float(ec.max()),
+                       "deflection_rel_err_median": float(np.median(ew)), "deflection_rel_err_max": float(ew.max()),
+                       "deflection_abs_err_max": float(np.abs(w - w0).max()),
+                       "alpha_error": float(m.alpha_error)}
+                log(f"[m2l] {row}")
+                rows.append(row)
+            m.close()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,11 +148,27 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-irs", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy kernel (scale for HBM byte counters)")
+    ap.add_argument("--m2l-stress", action="store_true",
+                    help="instead of the map benchmark: MFMA M2L against FP64 M2L on the 20000^2 FMM stress workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the map would be sharded by "
               f"lattice rows with one sum-reduction of the maps)", file=sys.stderr)
         sys.exit(2)
+
+    if args.m2l_stress:
+        print(json.dumps({"m2l_stress": m2l_stress((10, 20, 30))}), flush=True)
+        return
+    if args.copy_probe:
+        # a copy kernel of known size: 1 GiB read + 1 GiB written, for scaling FETCH_SIZE / WRITE_SIZE
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
 
     from irs_amd.microlensing.IPM.ipm import IPM
 
@@ -164,7 +245,8 @@ def main():
                    "num_stars": m.num_stars, "cells": cells, "lattice": list(m.num_ray_threads),
                    "expansion_order": P_ORDER, "tree_levels": m.tree_levels, "mean_near_stars": nbar},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": measured_traffic(),
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
                      "flop_per_cell": fpc, "kernel_ms": 1000.0 * float(np.mean(shoot_s)),
                      "peak_packed_fp32": PEAK_PACKED_FP32_TFLOPS},
         "shoot_rays_per_s": shoot_rate,

@@ -100,8 +100,8 @@ void set_expansion_order(IRSHandle* self, int val);
 int get_expansion_order(IRSHandle* self);
 void set_precision(IRSHandle* self, int val);
 int get_precision(IRSHandle* self);
-/* M2L of the FMM: 0 FP64 recurrences (default); 1 batched GEMMs on MFMA, FP32
-   for p <= 20 and FP64 above; 2 FP32 MFMA; 3 FP64 MFMA (fmm.cuh:281-374) */
+/* M2L of the FMM (fmm.cuh:281-374): 0 FP64 recurrences; 1 batched GEMMs on
+   MFMA, FP32 for p <= 20 and FP64 above; 2 FP32 MFMA; 3 FP64 MFMA (default) */
 void set_m2l_mode(IRSHandle* self, int val);
 int get_m2l_mode(IRSHandle* self);
 double get_t_m2l(IRSHandle* self);          /* seconds in the M2L kernels of the last build (MFMA modes) */

@@ -70,7 +70,7 @@ public:
 
 	/* extensions (no reference counterpart) */
 	int expansion_order_override = 0;   // 0: automatic order of ipm.cuh:848-860
-	int m2l_mode = M2L_DIRECT;          // M2LMode: FP64 recurrences or batched MFMA GEMMs
+	int m2l_mode = M2L_MFMA_F64;        // M2LMode: batched GEMMs on FP64 MFMA unless set otherwise
 	int kernel = KERNEL_FAST;
 	int precision = std::is_same<T, float>::value ? PREC_MIXED : PREC_FP64;
 
