@@ -138,6 +138,92 @@ def m2l_stress(orders, npoints=200000):
     return rows
 
 
+def batch_bench(n, seed=0):
+    """The heterogeneous batch: n maps drawn by draw_instances (seed), shot in one
+    graph launch, against the same instances shot one launch at a time."""
+    from irs_amd.microlensing.IPM.batch import IPMBatch, draw_instances
+    from irs_amd.microlensing.IPM.ipm import IPM
+    params = draw_instances(n, seed=seed)
+    t0 = time.perf_counter()
+    objs = []
+    for p in params:
+        m = IPM(**p)
+        m.prepare()
+        m.build()
+        objs.append(m)
+    t_build = time.perf_counter() - t0
+    batch = IPMBatch()
+    for m in objs:
+        batch.add(m)
+    log(f"[batch] {n} instances, {batch.cells:.3e} cells, prepare+build {t_build:.1f} s")
+    batch.run(copy_maps=False)
+    log(f"[batch] first run (autotune {batch.t_tune:.2f} s): tile_h {batch.tile_h}, blocks/CU {batch.blocks_per_cu}")
+    runs = []
+    for _ in range(3):
+        batch.run(copy_maps=False)
+        runs.append(batch.t_shoot)
+    t_batch = float(np.median(runs))
+    log(f"[batch] graph launch {t_batch:.3f} s ({batch.cells / t_batch:.3e} cells/s)")
+    t_single = 0.0
+    for m in objs:
+        m.shoot()
+        t_single += m.t_shoot_cells
+    log(f"[batch] one launch per instance: {t_single:.3f} s of kernels ({batch.cells / t_single:.3e} cells/s)")
+    out = {"instances": n, "seed": seed, "cells": batch.cells, "t_prepare_build_s": t_build,
+           "t_batch_graph_s": t_batch, "batch_cells_per_s": batch.cells / t_batch,
+           "t_single_kernels_s": t_single, "single_cells_per_s": batch.cells / t_single,
+           "tile_h": batch.tile_h, "blocks_per_cu": batch.blocks_per_cu, "t_tune_s": batch.t_tune}
+    batch.close()
+    for m in objs:
+        m.close()
+    return out
+
+
+def surrogate_bench(n_train=20000, n_cal=1024, n_test=2048, epochs=150, seed=11):
+    """The learned histogram surrogate: IPM histograms of n_train + n_cal +
+    n_test parameter draws (batched launches), training, entropy-threshold
+    calibration on n_cal draws, and KS distances / throughput on n_test
+    held-out draws against plain IPM and against the mean training histogram
+    (the method without the network)."""
+    from irs_amd.microlensing.Surrogate import data
+    from irs_amd.microlensing.Surrogate import surrogate as S
+    params = data.draw_params(n_train + n_cal + n_test, seed=seed)
+    t0 = time.perf_counter()
+    hist, t_ipm = data.histograms(params, log=log)
+    t_data = time.perf_counter() - t0
+    # the floor: the same test parameters with other star fields
+    test_params = params[n_train + n_cal:]
+    hist2, _ = data.histograms([dict(p, random_seed=p["random_seed"] + 7919) for p in test_params], log=log)
+    x = S.features(params)
+    tr, ca, te = slice(0, n_train), slice(n_train, n_train + n_cal), slice(n_train + n_cal, None)
+    s = S.HistogramSurrogate(max_batch=256, seed=1)
+    t0 = time.perf_counter()
+    losses = s.fit(x[tr], hist[tr], epochs=epochs, log=log)
+    t_train = time.perf_counter() - t0
+    thr = s.calibrate(x[ca], hist[ca], ks_target=0.02, quantile=0.5)
+    p = s.predict(x[te])
+    ks = S.ks_distance(p, hist[te])
+    acc = s.accept(p)
+    mean_h = S.normalise(hist[tr]).mean(axis=0)
+    ks_base = S.ks_distance(np.tile(mean_h, (len(p), 1)), hist[te])
+    ks_floor = S.ks_distance(hist2, hist[te])
+    t_pred = s.seconds_per_histogram(x[te])
+    t_map = t_ipm / len(params)
+    out = {"train": n_train, "calibration": n_cal, "test": n_test, "epochs": epochs, "t_data_s": t_data,
+           "t_ipm_batched_s": t_ipm, "t_train_s": t_train, "loss_first": losses[0], "loss_last": losses[-1],
+           "entropy_threshold": float(thr), "accepted_fraction": float(acc.mean()),
+           "ks_median": float(np.median(ks)), "ks_p95": float(np.percentile(ks, 95)), "ks_max": float(ks.max()),
+           "ks_accepted_median": float(np.median(ks[acc])) if acc.any() else None,
+           "ks_accepted_p95": float(np.percentile(ks[acc], 95)) if acc.any() else None,
+           "ks_mean_histogram_median": float(np.median(ks_base)), "ks_mean_histogram_p95": float(np.percentile(ks_base, 95)),
+           "ks_ipm_vs_ipm_median": float(np.median(ks_floor)), "ks_ipm_vs_ipm_p95": float(np.percentile(ks_floor, 95)),
+           "s_per_histogram_surrogate": t_pred, "s_per_histogram_ipm_batched": t_map,
+           "speedup_vs_ipm": t_map / t_pred}
+    log(f"[surrogate] {out}")
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +236,10 @@ def main():
     ap.add_argument("--no-irs", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy kernel (scale for HBM byte counters)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="instead of the map benchmark: this many heterogeneous maps in one graph launch")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="instead of the map benchmark: train and evaluate the histogram surrogate")
     ap.add_argument("--m2l-stress", action="store_true",
                     help="instead of the map benchmark: MFMA M2L against FP64 M2L on the 20000^2 FMM stress workload")
     args = ap.parse_args()
@@ -160,6 +250,12 @@ def main():
 
     if args.m2l_stress:
         print(json.dumps({"m2l_stress": m2l_stress((10, 20, 30))}), flush=True)
+        return
+    if args.batch:
+        print(json.dumps({"batch": batch_bench(args.batch)}), flush=True)
+        return
+    if args.surrogate:
+        print(json.dumps({"surrogate": surrogate_bench()}), flush=True)
         return
     if args.copy_probe:
         # a copy kernel of known size: 1 GiB read + 1 GiB written, for scaling FETCH_SIZE / WRITE_SIZE

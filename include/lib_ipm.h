@@ -149,6 +149,31 @@ bool ipm_deflect(IPMHandle* self, const double* z, int n, double* w, double* mu)
 int get_num_nodes(IPMHandle* self);
 bool ipm_export_tree(IPMHandle* self, double* geom, int* info, int* lists);
 bool ipm_export_coeffs(IPMHandle* self, int which, double* out);   /* 0 multipole, 1 local; 2(p+1) doubles per node */
+/* validation and derived geometry only (no device work): num_stars, the
+   lattice and the image-plane region become readable (ipm.cuh:370-565) */
+bool ipm_derive(IPMHandle* self, int verbose);
+
+/* Many IPM maps in one persistent, graph-captured launch (mixed precision,
+   no parity maps). Each instance is an object that has passed ipm_prepare
+   and ipm_build; ipm_batch_add allocates its map. ipm_batch_shoot zeroes every
+   map, shoots every instance and bins each map into a 128-bin histogram of
+   log10(magnification) over [-2, 2] (below/above go to the first/last bin);
+   ipm_finish on an instance then copies its map to the host. The first
+   shoot times candidate tile heights and resident blocks per CU on a sample
+   of the tiles and keeps the fastest. Extension: no reference counterpart. */
+typedef struct IPMBatch IPMBatch;
+IPMBatch* ipm_batch_create(void);
+void ipm_batch_free(IPMBatch* self);
+int ipm_batch_add(IPMBatch* self, IPMHandle* instance);        /* index, or -1 */
+bool ipm_batch_shoot(IPMBatch* self, int verbose);
+double get_batch_t_shoot(IPMBatch* self);                      /* seconds of the last graph launch */
+long long get_batch_cells(IPMBatch* self);
+int get_batch_size(IPMBatch* self);
+bool ipm_batch_histograms(IPMBatch* self, unsigned* out);      /* size x 128 counts */
+int get_batch_tile_h(IPMBatch* self);
+int get_batch_blocks_per_cu(IPMBatch* self);
+double get_batch_t_tune(IPMBatch* self);
+bool ipm_batch_set_tuning(IPMBatch* self, int tile_h, int blocks_per_cu);   /* fixed tiles, no autotuning */
 
 #ifdef __cplusplus
 }

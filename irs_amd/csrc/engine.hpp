@@ -44,6 +44,8 @@ enum ShootKernel
 	KERNEL_REFERENCE = 1,   // the reference's per-cell algorithm, FP64 (ipm_functions.cuh:112-230)
 };
 
+constexpr int BATCH_HIST_BINS = 128;   // log10 magnification histogram of a batched map, [-2, 2]
+
 template <typename T>
 struct DeviceBuffer
 {
@@ -127,6 +129,11 @@ public:
 	int map_pixels(int i) const { return npix_[i]; }
 	bool maps_are_fp32() const { return map_fp32_; }
 
+	/* kernel arguments of this engine's IPM map (mixed precision) for a batched launch;
+	   `out` receives the engine-internal argument block, batch_args_size() bytes */
+	static size_t batch_args_size();
+	void fill_batch_args(void* out, const LensParams& lens, const GridParams& g) const;
+
 private:
 	void ensure_node_capacity(int n);
 	void build_leaf_tables(const LensParams& lens);
@@ -187,6 +194,37 @@ private:
 	DeviceBuffer<int> scratch_i_;
 	DeviceBuffer<double> scratch_d_;
 	DeviceBuffer<long long> scratch_ll_;
+};
+
+/*
+Many IPM maps shot in one persistent launch. Each instance is an Engine whose
+tree and FMM are built and whose FP32 map is allocated; the batch zeroes the
+maps, shoots every instance's lattice tiles from one work queue and bins each
+map into a 128-bin log10 magnification histogram over [-2, 2], all captured
+in one HIP graph. The first shoot() times candidate tile heights and resident
+blocks per CU on a sample of the tiles and keeps the fastest.
+*/
+class BatchShooter
+{
+public:
+	BatchShooter();
+	~BatchShooter();
+	BatchShooter(const BatchShooter&) = delete;
+	BatchShooter& operator=(const BatchShooter&) = delete;
+
+	int add(Engine* e, const LensParams& lens, const GridParams& g);   // returns the instance index
+	double shoot(int verbose);                                          // ms of the graph launch
+	void histograms(unsigned* host) const;                              // size() x BATCH_HIST_BINS
+	int size() const;
+	long long cells() const;
+	int tile_h() const;
+	int blocks_per_cu() const;
+	double tune_ms() const;
+	void set_tuning(int tile_h, int blocks_per_cu);   // skip the autotuner
+
+private:
+	struct Impl;
+	Impl* impl_;
 };
 
 }  // namespace mlens

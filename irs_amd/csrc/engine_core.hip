@@ -36,6 +36,7 @@ template struct DeviceBuffer<int>;
 template struct DeviceBuffer<char>;
 template struct DeviceBuffer<long long>;
 template struct DeviceBuffer<unsigned long long>;
+template struct DeviceBuffer<unsigned short>;
 
 static inline unsigned blocks_for(long long n, int threads) { return unsigned((n + threads - 1) / threads); }
 

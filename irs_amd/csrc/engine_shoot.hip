@@ -14,7 +14,10 @@
 
 #include <algorithm>
 #include <cstring>
+#include <chrono>
+#include <iostream>
 #include <type_traits>
+#include <vector>
 
 namespace mlens
 {
@@ -774,28 +777,20 @@ constexpr int IPM_WAVES = 4;
 constexpr int IPM_TILE_W = 63 * IPM_WAVES;
 constexpr int IPM_TILE_H = 64;
 
+// One tile: this wave's 63 cells of lattice columns tile_x * IPM_TILE_W + wave * 63 + [0, 63),
+// cell rows [J0, J1). Every lane of the wave must call it.
 template <int PREC, bool PARITY>
-__global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long long row0, long long row1,
-	MLENS_TABLE_PARAMS, void* __restrict__ map, void* __restrict__ mmin, void* __restrict__ msad)
+__device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long long tile_x, long long J0, long long J1)
 {
-	MLENS_TABLE_REBIND(a)
-	a.map = map;
-	a.mmin = mmin;
-	a.msad = msad;
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
-	const long long I = (long long)blockIdx.x * IPM_TILE_W + wave * 63 + lane;
-	const long long J0 = row0 + (long long)blockIdx.y * IPM_TILE_H;
-	if (J0 >= row1) return;
-	const long long J1 = min(J0 + IPM_TILE_H, row1);
+	const long long I = tile_x * IPM_TILE_W + wave * 63 + lane;
 	const long long Ic = I < a.N1 ? I : a.N1;
 	const double zx = a.x0 + a.stepx * double(Ic);
 	const bool cell_ok = lane < 63 && I < a.N1;
 	LeafCache cache;
 	using MapT = typename std::conditional<PREC == PREC_MIXED, float, double>::type;
 	PixelAcc<MapT> acc;
-	__shared__ TriQueue queues[IPM_WAVES];
-	TriQueue& q = queues[wave];
 	int qn = 0;
 
 	auto corner = [&](long long J, double& px, double& py) {
@@ -838,6 +833,99 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long l
 		bry = try_;
 	}
 	acc_flush<MapT, PARITY>(a, acc);
+}
+
+template <int PREC, bool PARITY>
+__global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long long row0, long long row1,
+	MLENS_TABLE_PARAMS, void* __restrict__ map, void* __restrict__ mmin, void* __restrict__ msad)
+{
+	MLENS_TABLE_REBIND(a)
+	a.map = map;
+	a.mmin = mmin;
+	a.msad = msad;
+	const long long J0 = row0 + (long long)blockIdx.y * IPM_TILE_H;
+	if (J0 >= row1) return;
+	const long long J1 = min(J0 + IPM_TILE_H, row1);
+	__shared__ TriQueue queues[IPM_WAVES];
+	ipm_tile<PREC, PARITY>(a, queues[threadIdx.x >> 6], blockIdx.x, J0, J1);
+}
+
+/* ---------------------------------------------------------------------------
+Many maps in one launch. A persistent grid takes (instance, tile) work items
+from a global counter: item t covers tile t * stride of the concatenation of
+every instance's tiles (stride > 1 samples the batch for the autotuner). The
+instance of a tile comes from a binary search over the running tile counts;
+its arguments, including its own tables and map, are read from device memory.
+--------------------------------------------------------------------------- */
+struct BatchMap
+{
+	float* map;
+	long long npix;
+};
+
+__global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_batch(const ShootArgs* __restrict__ args,
+	const long long* __restrict__ tile_start, int ninst, int tile_h, long long total, long long stride,
+	unsigned long long* __restrict__ counter)
+{
+	__shared__ TriQueue queues[IPM_WAVES];
+	__shared__ long long next;
+	for (;;)
+	{
+		__syncthreads();   // every thread has read the previous item
+		if (threadIdx.x == 0) next = (long long)atomicAdd(counter, 1ull) * stride;
+		__syncthreads();
+		const long long t = next;
+		if (t >= total) break;
+		int lo = 0, hi = ninst - 1;
+		while (lo < hi)
+		{
+			const int mid = (lo + hi + 1) >> 1;
+			if (tile_start[mid] <= t)
+				lo = mid;
+			else
+				hi = mid - 1;
+		}
+		const ShootArgs a = args[lo];
+		const long long local = t - tile_start[lo];
+		const long long ntx = (a.N1 + IPM_TILE_W - 1) / IPM_TILE_W;
+		const long long ty = local / ntx, tx = local - ty * ntx;
+		const long long J0 = ty * tile_h, J1 = min(J0 + (long long)tile_h, a.N2);
+		ipm_tile<PREC_MIXED, false>(a, queues[threadIdx.x >> 6], tx, J0, J1);
+	}
+}
+
+// zero every instance's map (blockIdx.y = instance), its histogram and the work counter
+__global__ void k_batch_zero(const BatchMap* __restrict__ maps, unsigned* hist, unsigned long long* counter)
+{
+	const int inst = blockIdx.y;
+	if (blockIdx.x == 0)
+	{
+		for (int k = threadIdx.x; k < BATCH_HIST_BINS; k += blockDim.x) hist[(long long)inst * BATCH_HIST_BINS + k] = 0;
+		if (inst == 0 && threadIdx.x == 0) *counter = 0;
+	}
+	const BatchMap m = maps[inst];
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m.npix; i += (long long)gridDim.x * blockDim.x)
+		m.map[i] = 0.0f;
+}
+
+// 128-bin histogram of log10(magnification) over [-2, 2] per instance; values
+// below 10^-2 (zero included) count in the first bin, above 10^2 in the last
+__global__ void k_batch_hist(const BatchMap* __restrict__ maps, unsigned* hist)
+{
+	__shared__ unsigned h[BATCH_HIST_BINS];
+	for (int k = threadIdx.x; k < BATCH_HIST_BINS; k += blockDim.x) h[k] = 0;
+	__syncthreads();
+	const BatchMap m = maps[blockIdx.y];
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m.npix; i += (long long)gridDim.x * blockDim.x)
+	{
+		const float v = m.map[i];
+		int b = v > 0.0f ? int(floorf((log10f(v) + 2.0f) * (BATCH_HIST_BINS / 4.0f))) : 0;
+		b = min(max(b, 0), BATCH_HIST_BINS - 1);
+		atomicAdd(&h[b], 1u);
+	}
+	__syncthreads();
+	for (int k = threadIdx.x; k < BATCH_HIST_BINS; k += blockDim.x)
+		if (h[k]) atomicAdd(&hist[(long long)blockIdx.y * BATCH_HIST_BINS + k], h[k]);
 }
 
 /* ---------------------------------------------------------------------------
@@ -1689,6 +1777,236 @@ void Engine::map_histogram(int which, bool log_mode, int factor, long long hmin,
 	MLENS_HIP_CHECK(hipGetLastError());
 	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
 	MLENS_HIP_CHECK(hipMemcpy(host_hist, dh.ptr, sizeof(long long) * len, hipMemcpyDeviceToHost));
+}
+
+}  // namespace mlens
+
+namespace mlens
+{
+
+size_t Engine::batch_args_size() { return sizeof(ShootArgs); }
+
+void Engine::fill_batch_args(void* out, const LensParams& lens, const GridParams& g) const
+{
+	if (!map_fp32_ || map_int_ || !map_.ptr)
+		throw HipError("a batched instance needs its FP32 IPM map allocated", __FILE__, __LINE__);
+	ShootArgs a = ShootArgsBuilder::make(lens, g, false);
+	MLENS_TREE_ARGS(a)
+	a.map = map_.ptr;
+	a.mmin = nullptr;
+	a.msad = nullptr;
+	std::memcpy(out, &a, sizeof(a));
+}
+
+struct BatchShooter::Impl
+{
+	std::vector<Engine*> engines;
+	std::vector<ShootArgs> args;
+	std::vector<BatchMap> maps;
+	std::vector<long long> tiles_x;   // per instance: column tiles
+	bool dirty = true;
+	int tile_h = IPM_TILE_H, bpc = 3, ncu = 0;
+	double tune_ms = 0;
+	bool tuned = false;
+	hipStream_t stream = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	int graph_tile_h = -1, graph_bpc = -1;
+	DeviceBuffer<char> d_args, d_maps;
+	DeviceBuffer<long long> d_start;
+	DeviceBuffer<unsigned long long> d_counter;
+	DeviceBuffer<int> d_hist;
+	long long total = 0;
+	long long max_npix = 0;
+
+	long long tiles(int h) const
+	{
+		long long t = 0;
+		for (size_t i = 0; i < args.size(); i++) t += tiles_x[i] * ((args[i].N2 + h - 1) / h);
+		return t;
+	}
+
+	void upload_start(int h)
+	{
+		std::vector<long long> st(args.size() + 1, 0);
+		for (size_t i = 0; i < args.size(); i++) st[i + 1] = st[i] + tiles_x[i] * ((args[i].N2 + h - 1) / h);
+		total = st.back();
+		d_start.alloc(st.size());
+		MLENS_HIP_CHECK(hipMemcpy(d_start.ptr, st.data(), st.size() * sizeof(long long), hipMemcpyHostToDevice));
+	}
+
+	void upload()
+	{
+		const size_t n = args.size();
+		d_args.alloc(n * sizeof(ShootArgs));
+		MLENS_HIP_CHECK(hipMemcpy(d_args.ptr, args.data(), n * sizeof(ShootArgs), hipMemcpyHostToDevice));
+		d_maps.alloc(n * sizeof(BatchMap));
+		MLENS_HIP_CHECK(hipMemcpy(d_maps.ptr, maps.data(), n * sizeof(BatchMap), hipMemcpyHostToDevice));
+		d_counter.alloc(1);
+		d_hist.alloc(n * BATCH_HIST_BINS);
+		dirty = false;
+	}
+
+	// zero kernel launch on the batch stream (graph-safe: no memset nodes)
+	void launch_zero()
+	{
+		const unsigned bx = unsigned(std::min<long long>(1024, std::max<long long>(1, (max_npix + 255) / 256)));
+		dim3 grid(bx, unsigned(args.size()));
+		k_batch_zero<<<grid, 256, 0, stream>>>(reinterpret_cast<const BatchMap*>(d_maps.ptr),
+			reinterpret_cast<unsigned*>(d_hist.ptr), d_counter.ptr);
+		MLENS_HIP_CHECK(hipGetLastError());
+	}
+
+	void launch_shoot(int h, int blocks_per_cu, long long stride)
+	{
+		const unsigned blocks = unsigned(ncu * blocks_per_cu);
+		k_ipm_batch<<<blocks, 64 * IPM_WAVES, 0, stream>>>(reinterpret_cast<const ShootArgs*>(d_args.ptr), d_start.ptr,
+			int(args.size()), h, total, stride, d_counter.ptr);
+		MLENS_HIP_CHECK(hipGetLastError());
+	}
+
+	void launch_hist()
+	{
+		const unsigned bx = unsigned(std::min<long long>(256, std::max<long long>(1, (max_npix + 1023) / 1024)));
+		dim3 grid(bx, unsigned(args.size()));
+		k_batch_hist<<<grid, 256, 0, stream>>>(reinterpret_cast<const BatchMap*>(d_maps.ptr),
+			reinterpret_cast<unsigned*>(d_hist.ptr));
+		MLENS_HIP_CHECK(hipGetLastError());
+	}
+
+	float elapsed()
+	{
+		float ms = 0;
+		MLENS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+		return ms;
+	}
+
+	void autotune(int verbose)
+	{
+		// about an eighth of the tiles, spread over every instance
+		const long long stride = 8;
+		const int heights[3] = {32, 64, 128};
+		const int resident[3] = {2, 3, 4};
+		float best = -1;
+		auto t0 = std::chrono::steady_clock::now();
+		for (int h : heights)
+		{
+			upload_start(h);
+			for (int b : resident)
+			{
+				launch_zero();
+				MLENS_HIP_CHECK(hipEventRecord(e0, stream));
+				launch_shoot(h, b, stride);
+				MLENS_HIP_CHECK(hipEventRecord(e1, stream));
+				MLENS_HIP_CHECK(hipEventSynchronize(e1));
+				const float ms = elapsed();
+				if (verbose >= 2) std::cout << "batch tune: tile_h " << h << ", blocks/CU " << b << ": " << ms << " ms\n";
+				if (best < 0 || ms < best)
+				{
+					best = ms;
+					tile_h = h;
+					bpc = b;
+				}
+			}
+		}
+		tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		tuned = true;
+	}
+
+	void capture()
+	{
+		if (exec) (void)hipGraphExecDestroy(exec);
+		if (graph) (void)hipGraphDestroy(graph);
+		exec = nullptr;
+		graph = nullptr;
+		upload_start(tile_h);
+		MLENS_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+		launch_zero();
+		launch_shoot(tile_h, bpc, 1);
+		launch_hist();
+		MLENS_HIP_CHECK(hipStreamEndCapture(stream, &graph));
+		MLENS_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+		graph_tile_h = tile_h;
+		graph_bpc = bpc;
+	}
+};
+
+BatchShooter::BatchShooter() : impl_(new Impl)
+{
+	MLENS_HIP_CHECK(hipStreamCreateWithFlags(&impl_->stream, hipStreamNonBlocking));
+	MLENS_HIP_CHECK(hipEventCreate(&impl_->e0));
+	MLENS_HIP_CHECK(hipEventCreate(&impl_->e1));
+	int dev = 0;
+	MLENS_HIP_CHECK(hipGetDevice(&dev));
+	MLENS_HIP_CHECK(hipDeviceGetAttribute(&impl_->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+}
+
+BatchShooter::~BatchShooter()
+{
+	if (impl_->stream) (void)hipStreamSynchronize(impl_->stream);
+	if (impl_->exec) (void)hipGraphExecDestroy(impl_->exec);
+	if (impl_->graph) (void)hipGraphDestroy(impl_->graph);
+	if (impl_->e0) (void)hipEventDestroy(impl_->e0);
+	if (impl_->e1) (void)hipEventDestroy(impl_->e1);
+	if (impl_->stream) (void)hipStreamDestroy(impl_->stream);
+	delete impl_;
+}
+
+int BatchShooter::add(Engine* e, const LensParams& lens, const GridParams& g)
+{
+	ShootArgs a;
+	e->fill_batch_args(&a, lens, g);
+	if (a.N1 <= 0 || a.N2 <= 0) throw HipError("a batched instance has an empty lattice", __FILE__, __LINE__);
+	impl_->engines.push_back(e);
+	impl_->args.push_back(a);
+	impl_->maps.push_back(BatchMap{reinterpret_cast<float*>(a.map), (long long)a.n1 * a.n2});
+	impl_->tiles_x.push_back((a.N1 + IPM_TILE_W - 1) / IPM_TILE_W);
+	impl_->max_npix = std::max(impl_->max_npix, (long long)a.n1 * a.n2);
+	impl_->dirty = true;
+	impl_->graph_tile_h = -1;
+	return int(impl_->args.size()) - 1;
+}
+
+double BatchShooter::shoot(int verbose)
+{
+	Impl& m = *impl_;
+	if (m.args.empty()) return 0;
+	// the instances' engines built their tables on their own streams
+	for (Engine* e : m.engines) MLENS_HIP_CHECK(hipStreamSynchronize(e->stream()));
+	if (m.dirty) m.upload();
+	if (!m.tuned) m.autotune(verbose);
+	if (m.graph_tile_h != m.tile_h || m.graph_bpc != m.bpc || !m.exec) m.capture();
+	MLENS_HIP_CHECK(hipEventRecord(m.e0, m.stream));
+	MLENS_HIP_CHECK(hipGraphLaunch(m.exec, m.stream));
+	MLENS_HIP_CHECK(hipEventRecord(m.e1, m.stream));
+	MLENS_HIP_CHECK(hipEventSynchronize(m.e1));
+	return m.elapsed();
+}
+
+void BatchShooter::histograms(unsigned* host) const
+{
+	const size_t n = impl_->args.size() * BATCH_HIST_BINS;
+	if (n) MLENS_HIP_CHECK(hipMemcpy(host, impl_->d_hist.ptr, n * sizeof(unsigned), hipMemcpyDeviceToHost));
+}
+
+int BatchShooter::size() const { return int(impl_->args.size()); }
+long long BatchShooter::cells() const
+{
+	long long c = 0;
+	for (const ShootArgs& a : impl_->args) c += a.N1 * a.N2;
+	return c;
+}
+int BatchShooter::tile_h() const { return impl_->tile_h; }
+int BatchShooter::blocks_per_cu() const { return impl_->bpc; }
+double BatchShooter::tune_ms() const { return impl_->tune_ms; }
+void BatchShooter::set_tuning(int tile_h, int blocks_per_cu)
+{
+	if (tile_h < 1 || tile_h > 4096 || blocks_per_cu < 1 || blocks_per_cu > 16)
+		throw HipError("batch tuning outside 1..4096 rows and 1..16 blocks per CU", __FILE__, __LINE__);
+	impl_->tile_h = tile_h;
+	impl_->bpc = blocks_per_cu;
+	impl_->tuned = true;
 }
 
 }  // namespace mlens

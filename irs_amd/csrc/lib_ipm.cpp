@@ -30,6 +30,7 @@ ipm_dtype* get_pixels_saddles(IPMHandle* self) { return self->m.get_pixels_saddl
 double get_t_shoot_cells(IPMHandle* self) { return self->m.get_t_shoot(); }
 
 bool ipm_prepare(IPMHandle* self, int verbose) { MLENS_GUARD(return self->m.prepare(verbose), return false) }
+bool ipm_derive(IPMHandle* self, int verbose) { MLENS_GUARD(return self->m.derive(verbose), return false) }
 bool ipm_build(IPMHandle* self, int verbose) { MLENS_GUARD(return self->m.build(verbose), return false) }
 bool ipm_shoot(IPMHandle* self, int verbose, long long row0, long long row1)
 {
@@ -54,6 +55,33 @@ bool ipm_export_coeffs(IPMHandle* self, int which, double* out)
 {
 	if (!self->m.engine()) return false;
 	MLENS_GUARD(self->m.engine()->export_coeffs(which, out); return true, return false)
+}
+
+/* many maps in one launch */
+struct IPMBatch
+{
+	mlens::BatchShooter b;
+	double t = 0;
+};
+
+IPMBatch* ipm_batch_create() { MLENS_GUARD(return new IPMBatch(), return nullptr) }
+void ipm_batch_free(IPMBatch* self) { delete self; }
+int ipm_batch_add(IPMBatch* self, IPMHandle* m)
+{
+	MLENS_GUARD(if (!m->m.prepare_batch_map()) return -1; return self->b.add(m->m.engine(), m->m.lens(), m->m.grid()),
+		return -1)
+}
+bool ipm_batch_shoot(IPMBatch* self, int verbose) { MLENS_GUARD(self->t = self->b.shoot(verbose) / 1000.0; return true, return false) }
+double get_batch_t_shoot(IPMBatch* self) { return self->t; }
+long long get_batch_cells(IPMBatch* self) { return self->b.cells(); }
+int get_batch_size(IPMBatch* self) { return self->b.size(); }
+bool ipm_batch_histograms(IPMBatch* self, unsigned* out) { MLENS_GUARD(self->b.histograms(out); return true, return false) }
+int get_batch_tile_h(IPMBatch* self) { return self->b.tile_h(); }
+int get_batch_blocks_per_cu(IPMBatch* self) { return self->b.blocks_per_cu(); }
+double get_batch_t_tune(IPMBatch* self) { return self->b.tune_ms() / 1000.0; }
+bool ipm_batch_set_tuning(IPMBatch* self, int tile_h, int blocks_per_cu)
+{
+	MLENS_GUARD(self->b.set_tuning(tile_h, blocks_per_cu); return true, return false)
 }
 
 }  // extern "C"

@@ -97,6 +97,32 @@ public:
 		}
 	}
 
+	// validation and derived geometry only, nothing on the device (num_stars,
+	// lattice and the image-plane region become available)
+	bool derive(int verbose)
+	{
+		prepared_ = false;
+		built_ = false;
+		return check_input_params(verbose) && calculate_derived_params(verbose);
+	}
+
+	// allocates the FP32 map a batched launch writes (needs build() and mixed precision)
+	bool prepare_batch_map()
+	{
+		if (!built_)
+		{
+			std::cerr << "Error. build() must succeed before the map is batched\n";
+			return false;
+		}
+		if (!IS_IPM || precision != PREC_MIXED || write_parities)
+		{
+			std::cerr << "Error. batched maps are IPM maps in mixed precision (precision 1) without parities\n";
+			return false;
+		}
+		engine_->alloc_maps(num_pixels_y.re, num_pixels_y.im, true, PREC_MIXED, false);
+		return true;
+	}
+
 	bool build(int verbose)
 	{
 		try

@@ -498,6 +498,12 @@ class IPM(object):
             raise Exception("Error running IPM")
         self._copy_results()
 
+    def derive(self):
+        """Validate and derive the geometry without touching the GPU: num_stars,
+        the lattice and the image-plane region become readable."""
+        if not self.lib.ipm_derive(self.obj, self.verbose):
+            raise Exception("Error deriving IPM parameters")
+
     def prepare(self):
         if not self.lib.ipm_prepare(self.obj, self.verbose):
             raise Exception("Error preparing IPM")

@@ -1,0 +1,81 @@
+"""Training and evaluation data for the histogram surrogate: parameter draws
+and their IPM histograms, shot with the batched launch.
+
+Every draw fixes the map geometry (square map of `side` pixels over
+half-length `half_length` Einstein radii, `num_rays_y` rays per pixel) and
+draws kappa_tot ~ U(0.1, 0.9), shear ~ U(0, 0.9), smooth fraction ~ U(0, 0.9),
+a mass function among Kroupa, Salpeter and uniform, m_lower log-uniform in
+[0.01, 0.3] and m_upper log-uniform in [1, 10] solar masses. Draws within
+min_gap of the critical lines |1 - kappa -+ shear| = 0, or whose star field
+or lattice exceeds the caps, are redrawn. The FMM order is fixed (12 by
+default): the automatic order of small fields can fall below 3.
+
+The default map spans +-50 Einstein radii: the histogram of one map is one
+realisation of the star field, and two realisations with the same parameters
+differ by a KS distance of median 0.015 there, against 0.155 at +-5 (measured
+on 48 parameter draws, 256^2 pixels). A surrogate of the expected histogram
+cannot agree with single maps better than that.
+"""
+
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from ..IPM.batch import IPMBatch
+from ..IPM.ipm import IPM
+
+
+def draw_params(n, seed=0, side=256, half_length=50.0, num_rays_y=4, min_gap=0.05, max_stars=1.5e6, max_cells=4.0e8,
+                expansion_order=12):
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        kappa = float(rng.uniform(0.1, 0.9))
+        shear = float(rng.uniform(0.0, 0.9))
+        smooth = float(rng.uniform(0.0, 0.9))
+        if abs(1 - kappa - shear) < min_gap or abs(1 - kappa + shear) < min_gap:
+            continue
+        p = dict(kappa_tot=kappa, shear=shear, kappa_star=(1 - smooth) * kappa,
+                 mass_function=str(rng.choice(["kroupa", "salpeter", "uniform"])),
+                 m_lower=float(np.exp(rng.uniform(np.log(0.01), np.log(0.3)))),
+                 m_upper=float(np.exp(rng.uniform(np.log(1.0), np.log(10.0)))),
+                 half_length_y1=half_length, half_length_y2=half_length, num_pixels_y1=side, num_pixels_y2=side,
+                 num_rays_y=num_rays_y, random_seed=int(rng.integers(1, 2 ** 31 - 1)), write_stars=False,
+                 write_maps=False, write_histograms=False, precision=1, expansion_order=expansion_order)
+        m = IPM(**p)
+        m.derive()
+        n1, n2 = m.num_ray_threads
+        ok = m.num_stars <= max_stars and float(n1) * n2 <= max_cells
+        m.close()
+        if ok:
+            out.append(p)
+    return out
+
+
+def _build(m):
+    m.prepare()
+    m.build()
+
+
+def histograms(params, chunk=256, threads=8, tuning=(64, 3), log=None):
+    """(n, 128) histogram counts of the parameter draws, and the seconds spent
+    in the batched launches."""
+    out = np.zeros((len(params), 128), dtype=np.uint32)
+    t_shoot = 0.0
+    with ThreadPoolExecutor(threads) as pool:
+        for c0 in range(0, len(params), chunk):
+            objs = [IPM(**p) for p in params[c0:c0 + chunk]]
+            list(pool.map(_build, objs))
+            b = IPMBatch()
+            if tuning:
+                b.set_tuning(*tuning)
+            for m in objs:
+                b.add(m)
+            out[c0:c0 + len(objs)] = b.run(copy_maps=False)
+            t_shoot += b.t_shoot
+            b.close()
+            for m in objs:
+                m.close()
+            if log:
+                log(f"[data] {c0 + len(objs)}/{len(params)} histograms")
+    return out, t_shoot

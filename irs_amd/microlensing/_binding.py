@@ -94,6 +94,19 @@ def load(path, dtype, irs=False):
         sig("ipm_deflect", [v, P(ctypes.c_double), i, P(ctypes.c_double), P(ctypes.c_double)], b)
         sig("ipm_export_tree", [v, P(ctypes.c_double), P(i), P(i)], b)
         sig("ipm_export_coeffs", [v, i, P(ctypes.c_double)], b)
+        sig("ipm_derive", [v, i], b)
+        sig("ipm_batch_create", [], v)
+        sig("ipm_batch_free", [v], None)
+        sig("ipm_batch_add", [v, v], i)
+        sig("ipm_batch_shoot", [v, i], b)
+        sig("get_batch_t_shoot", [v], ctypes.c_double)
+        sig("get_batch_cells", [v], ll)
+        sig("get_batch_size", [v], i)
+        sig("ipm_batch_histograms", [v, P(ctypes.c_uint)], b)
+        sig("get_batch_tile_h", [v], i)
+        sig("get_batch_blocks_per_cu", [v], i)
+        sig("get_batch_t_tune", [v], ctypes.c_double)
+        sig("ipm_batch_set_tuning", [v, i, i], b)
     return lib
 
 
@@ -108,7 +121,9 @@ IPM_SYMBOLS = (["init"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f"get_{n}" fo
                 "get_num_ray_threads_2", "get_root_half_length", "get_mean_near_stars", "get_taylor_smooth"] + _DERIVED +
                ["ipm_prepare", "ipm_build",
                 "ipm_shoot", "ipm_finish", "get_cells_shot", "ipm_deflect", "get_num_nodes", "ipm_export_tree",
-                "ipm_export_coeffs"] + _M2L_SYMBOLS)
+                "ipm_export_coeffs", "ipm_derive", "ipm_batch_create", "ipm_batch_free", "ipm_batch_add",
+                "ipm_batch_shoot", "get_batch_t_shoot", "get_batch_cells", "get_batch_size", "ipm_batch_histograms",
+                "get_batch_tile_h", "get_batch_blocks_per_cu", "get_batch_t_tune", "ipm_batch_set_tuning"] + _M2L_SYMBOLS)
 
 IRS_SYMBOLS = (["init", "irs_free"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f"get_{n}" for n, _ in _FIELDS[:26]] +
                ["get_num_stars", "get_corner_x1", "get_corner_x2", "get_stars", "get_pixels", "get_pixels_minima",

@@ -300,3 +300,41 @@ def test_saved_files_read_back(tmp_path, c1_stars, is_double):
     ref = np.bincount(np.floor(v + 0.5).astype(np.int64))
     assert np.array_equal(h[:, 1], ref[ref > 0])
     m.close()
+
+
+def test_batched_maps_match_single_runs():
+    """Several heterogeneous maps in one graph-captured persistent launch equal
+    the same instances shot one by one (FP32 map atomics: summation order
+    only), and each histogram is the 128-bin log10 histogram of its map."""
+    from irs_amd.microlensing.IPM.batch import HIST_BINS, IPMBatch, draw_instances
+    from irs_amd.microlensing.IPM.ipm import IPM
+    params = draw_instances(6, seed=3, sides=(64, 128, 192), star_range=(1.5e3, 4.0e3), num_rays_y=3)
+    batch = IPMBatch()
+    objs = []
+    for p in params:
+        m = IPM(**p)
+        m.prepare()
+        m.build()
+        batch.add(m)
+        objs.append(m)
+    say(f"[gpu] batch of {len(objs)} maps, {batch.cells} cells")
+    for rep in range(2):   # the first run autotunes, the second replays the graph
+        h = batch.run()
+        say(f"[gpu] batch run {rep}: {batch.t_shoot:.4f} s, tile_h {batch.tile_h}, blocks/CU {batch.blocks_per_cu}")
+        for k, (p, m) in enumerate(zip(params, objs)):
+            single = IPM(**p)
+            single.run()
+            ref = single.magnifications.astype(np.float64)
+            got = m.magnifications.astype(np.float64)
+            e = rel_err(got, ref, floor=1e-3 * max(ref.max(), 1e-30))
+            assert np.max(e) < 2e-5, (k, np.max(e))
+            assert h[k].sum() == p["num_pixels_y1"] * p["num_pixels_y2"]
+            lv = np.where(got > 0, np.log10(np.maximum(got, 1e-300)), -np.inf)
+            b = np.clip(np.floor((lv + 2.0) * (HIST_BINS / 4.0)), 0, HIST_BINS - 1).astype(int)
+            ref_h = np.bincount(b.ravel(), minlength=HIST_BINS)
+            # device log10f and numpy may round a value at a bin edge differently
+            assert np.abs(h[k].astype(int) - ref_h).sum() <= 4, (k, np.abs(h[k].astype(int) - ref_h).sum())
+            single.close()
+    batch.close()
+    for m in objs:
+        m.close()

@@ -1,0 +1,159 @@
+"""The histogram surrogate network (irs_amd/csrc/surrogate.hip) against its
+CPU FP64 restatement (oracle/mlp.py), whose backward pass is itself pinned by
+finite differences."""
+
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    offs, n = mlp.layout()
+    p = np.zeros(n)
+    for l, (w, b) in enumerate(offs):
+        fin = mlp.NIN if l == 0 else mlp.DIMS[l]
+        W = np.zeros((mlp.DIMS[l + 1], mlp.DIMS[l]))
+        W[:, :fin] = rng.uniform(-1, 1, (mlp.DIMS[l + 1], fin)) * np.sqrt(6.0 / fin)
+        p[w:w + W.size] = W.ravel()
+        p[b:b + mlp.DIMS[l + 1]] = rng.normal(0, 0.1, mlp.DIMS[l + 1])
+    return p
+
+
+def random_batch(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, mlp.NIN))
+    z = rng.normal(size=(n, 128))
+    t = np.exp(z) / np.exp(z).sum(axis=1, keepdims=True)
+    return x, t
+
+
+def test_oracle_gradients_match_finite_differences():
+    p = random_params(1)
+    x, t = random_batch(8, 2)
+    _, g = mlp.gradients(p, x, t)
+    rng = np.random.default_rng(3)
+    offs, n = mlp.layout()
+    # parameters of every layer, weights and biases, excluding the zero-padded input columns
+    idx = []
+    for l, (w, b) in enumerate(offs):
+        fin = mlp.NIN if l == 0 else mlp.DIMS[l]
+        for _ in range(6):
+            o, i = rng.integers(mlp.DIMS[l + 1]), rng.integers(fin)
+            idx.append(w + o * mlp.DIMS[l] + i)
+        idx.append(b + rng.integers(mlp.DIMS[l + 1]))
+    h = 1e-6
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (mlp.loss(pp, x, t) - mlp.loss(pm, x, t)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-7 + 1e-5 * abs(fd), (k, fd, g[k])
+
+
+def test_bf16_rounding_known_answers():
+    one = 1.0
+    assert mlp.bf16(one) == 1.0
+    assert mlp.bf16(1 + 2 ** -8) == 1.0                    # tie, rounds to even
+    assert mlp.bf16(1 + 3 * 2 ** -8) == 1 + 2 ** -6        # tie, rounds up to even
+    assert mlp.bf16(1 + 2 ** -8 + 2 ** -12) == 1 + 2 ** -7
+    assert mlp.bf16(-3.0) == -3.0
+
+
+def test_ks_distance_and_features():
+    from irs_amd.microlensing.Surrogate.surrogate import features, ks_distance
+    a = np.zeros((1, 128))
+    a[0, 10] = 1
+    b = np.zeros((1, 128))
+    b[0, 100] = 1
+    assert ks_distance(a, a)[0] == 0 and ks_distance(a, b)[0] == 1
+    f = features([dict(kappa_tot=0.5, shear=0.2, kappa_star=0.4, mass_function="kroupa", m_lower=0.08, m_upper=1.5)])
+    assert np.allclose(f[0], [0.5, 0.2, 0.2, -2.3, 0.08, 1.5])
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_match_oracle():
+    from irs_amd.microlensing.Surrogate.surrogate import F, lib
+    h = lib.surrogate_create(64, 5)
+    n = lib.surrogate_num_params(h)
+    p = random_params(4).astype(np.float32)
+    assert n == p.size
+    assert lib.surrogate_set_params(h, p.ctypes.data_as(F))
+    back = np.zeros(n, dtype=np.float32)
+    lib.surrogate_get_params(h, back.ctypes.data_as(F))
+    assert np.array_equal(back, p)
+    x, t = random_batch(32, 6)
+    xf = np.ascontiguousarray(x, dtype=np.float32)
+    tf = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(n, dtype=np.float32)
+    loss = lib.surrogate_gradients(h, xf.ctypes.data_as(F), tf.ctypes.data_as(F), 32, g.ctypes.data_as(F))
+    L, gref = mlp.gradients(p.astype(np.float64), xf.astype(np.float64), tf.astype(np.float64), bf16_ops=True)
+    print(f"[gpu] surrogate loss {loss:.6f} oracle {L:.6f}", flush=True)
+    assert abs(loss - L) < 1e-4 * abs(L)
+    offs, _ = mlp.layout()
+    for l, (w, b) in enumerate(offs):
+        for lo, hi in ((w, b), (b, b + mlp.DIMS[l + 1])):
+            e = np.linalg.norm(g[lo:hi] - gref[lo:hi]) / max(np.linalg.norm(gref[lo:hi]), 1e-30)
+            print(f"[gpu] layer {l} [{lo}:{hi}] relative gradient error {e:.2e}", flush=True)
+            # FP32 accumulation against FP64 on identical BF16 operands; an operand
+            # rounded across a BF16 tie differently moves a few entries by one BF16 ulp
+            assert e < 3e-3
+    probs = np.zeros((32, 128), dtype=np.float32)
+    assert lib.surrogate_predict(h, xf.ctypes.data_as(F), 32, probs.ctypes.data_as(F))
+    pref, _, _ = mlp.forward(p.astype(np.float64), xf.astype(np.float64), bf16_ops=True)
+    assert np.max(np.abs(probs - pref)) < 1e-4
+    lib.surrogate_free(h)
+
+
+@pytest.mark.gpu
+def test_gpu_training_fits_a_smooth_family():
+    """Targets that are discretised Gaussians in log10 mu whose mean and width
+    depend smoothly on the inputs: training drives the loss down to near the
+    targets' own entropy and held-out predictions close to the targets."""
+    from irs_amd.microlensing.Surrogate.surrogate import HistogramSurrogate, ks_distance
+    rng = np.random.default_rng(0)
+
+    def family(x):
+        c = np.linspace(-2, 2, 129)[:-1] + 2 / 128
+        mu = 0.8 * np.tanh(x[:, 0]) + 0.3 * x[:, 1] - 0.2 * x[:, 2] * x[:, 3]
+        sd = 0.25 + 0.1 * (1 + np.tanh(x[:, 4]))
+        h = np.exp(-0.5 * ((c[None, :] - mu[:, None]) / sd[:, None]) ** 2)
+        return h / h.sum(axis=1, keepdims=True)
+
+    x = rng.normal(size=(4096, 6))
+    t = family(x)
+    s = HistogramSurrogate(max_batch=256, seed=2)
+    losses = s.fit(x, t, epochs=60, batch=256, lr=2e-3)
+    xt = rng.normal(size=(512, 6))
+    ks = ks_distance(s.predict(xt), family(xt))
+    ent = float(np.mean(-np.sum(t * np.log(np.clip(t, 1e-12, 1)), axis=1)))
+    print(f"[gpu] surrogate synthetic: loss {losses[0]:.4f} -> {losses[-1]:.4f} (target entropy {ent:.4f}), "
+          f"held-out KS median {np.median(ks):.4f} p95 {np.percentile(ks, 95):.4f}", flush=True)
+    assert losses[-1] < losses[0]
+    assert losses[-1] - ent < 0.02
+    assert np.median(ks) < 0.02 and np.percentile(ks, 95) < 0.05
+    s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_beats_mean_histogram_on_ipm_maps():
+    """On real IPM histograms (small maps), the trained network predicts
+    held-out histograms much better than the method without it (the mean
+    training histogram), at a fraction of the cost of shooting the maps."""
+    from irs_amd.microlensing.Surrogate import data
+    from irs_amd.microlensing.Surrogate import surrogate as S
+    params = data.draw_params(1024, seed=5, side=64, half_length=3.0, num_rays_y=3, max_stars=3e4, max_cells=3e7)
+    hist, t_ipm = data.histograms(params)
+    x = S.features(params)
+    s = S.HistogramSurrogate(max_batch=256, seed=3)
+    s.fit(x[:768], hist[:768], epochs=80, batch=256, lr=2e-3)
+    p = s.predict(x[768:])
+    ks = S.ks_distance(p, hist[768:])
+    base = S.ks_distance(np.tile(S.normalise(hist[:768]).mean(axis=0), (len(p), 1)), hist[768:])
+    t_pred = s.seconds_per_histogram(x[768:])
+    print(f"[gpu] surrogate on IPM maps: KS median {np.median(ks):.4f} (mean histogram {np.median(base):.4f}), "
+          f"{t_pred * 1e6:.2f} us per histogram vs {t_ipm / len(params) * 1e3:.3f} ms per batched map", flush=True)
+    assert np.median(ks) < 0.5 * np.median(base)
+    assert t_pred < t_ipm / len(params)
+    s.close()
