@@ -1,0 +1,77 @@
+"""The command-line programs irs_amd/bin/ipm, ipm_double and irs: the
+reference's options, messages and exit codes (src/main_ipm_irs.cu:41-606)."""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "irs_amd", "bin")
+
+
+def run(prog, *args, cwd=None):
+    r = subprocess.run([os.path.join(BIN, prog), *map(str, args)], capture_output=True, text=True, cwd=cwd)
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.mark.parametrize("prog", ["ipm", "ipm_double", "irs"])
+def test_help_and_syntax(prog):
+    rc, out, _ = run(prog, "-h")
+    assert rc == 255 and "Usage:" in out and "--kappa_tot" in out and "--outfile_prefix" in out
+    rc, _, err = run(prog, "-k")
+    assert rc == 255 and "Error. Invalid input syntax." in err
+    rc, _, err = run(prog, "-zz", "1")
+    assert rc == 255 and "Unknown option -zz" in err
+    rc, _, err = run(prog, "-k", "abc")
+    assert rc == 255 and "Error. Invalid kappa_tot input." in err
+    rc, _, err = run(prog, "-s", "1.5")
+    assert rc == 255 and "smooth_fraction must be < 1" in err
+    rc, _, err = run(prog, "-s", "-0.5")
+    assert rc == 255 and "smooth_fraction must be >= 0" in err
+
+
+def test_defaults_in_help():
+    # ipm.cuh:42-64 and irs.cuh:50,58
+    _, out, _ = run("ipm", "-h")
+    assert "light_loss. Default value: 0.001" in out and "num_rays_y. Default value: 1" in out
+    _, out, _ = run("irs", "-h")
+    assert "light_loss. Default value: 0.01" in out and "num_rays_y. Default value: 100" in out
+
+
+def test_option_semantics(gpu_available):
+    if gpu_available:
+        pytest.skip("checks the no-device path")
+    rc, out, err = run("ipm", "-v", "2", "-k", "0.45", "-s", "0.2", "-mf", "Kroupa", "-npy1", "10", "-npy2", "10")
+    assert "mass_function set to: kroupa" in out
+    assert "kappa_star set to: 0.36" in out
+    assert "No HIP capable devices" in err and rc == 255
+    # -ks wins over -s; a star file wins over every star-field option
+    _, out, _ = run("ipm", "-v", "2", "-s", "0.5", "-ks", "0.1")
+    assert "kappa_star set to: 0.1" in out and "smooth_fraction set to" not in out
+    _, out, _ = run("ipm", "-v", "2", "-sf", "stars.bin", "-ks", "0.1", "-mf", "salpeter", "-k", "0.4")
+    assert "kappa_star set to" not in out and "mass_function set to" not in out and "kappa_tot set to: 0.4" in out
+
+
+@pytest.mark.gpu
+def test_cli_map_matches_library(tmp_path):
+    """ipm writes the same map the library computes for the same parameters."""
+    from irs_amd.microlensing.IPM.ipm import IPM
+    from irs_amd.microlensing.Util import util
+    args = ["-k", "0.3", "-y", "0.3", "-ks", "0.3", "-hly1", "5", "-hly2", "5", "-npy1", "100", "-npy2", "100",
+            "-nry", "4", "-rs", "5", "-ws", "1", "-wm", "1", "-wh", "1", "-o", str(tmp_path) + "/", "-v", "0",
+            "--expansion_order", "12"]
+    print("[gpu] cli ipm_double", flush=True)
+    rc, out, err = run("ipm_double", *args)
+    assert rc == 0, err
+    got = util.read_array(str(tmp_path / "ipm_magnifications.bin"), np.float64)
+    stars, _, _, _ = util.read_stars(str(tmp_path / "ipm_stars.bin"), dtype=np.float64)
+    m = IPM(kappa_tot=0.3, shear=0.3, kappa_star=0.3, half_length_y1=5, half_length_y2=5, num_pixels_y1=100,
+            num_pixels_y2=100, num_rays_y=4, random_seed=5, write_stars=False, write_maps=False, write_histograms=False,
+            is_double=True, expansion_order=12)
+    m.run()
+    assert np.array_equal(stars, m.stars)
+    e = np.abs(got - m.magnifications) / np.maximum(np.abs(m.magnifications), 1e-12)
+    assert np.max(e) < 1e-10
+    m.close()
