@@ -464,6 +464,7 @@ struct TriQueue
 	int hkey[IPM_QCAP];                          // box height | map bit << 29 | needle << 30
 	int end[IPM_QCAP];                           // box pixels, then running total
 	float rw[IPM_QCAP];                          // 1 / w
+	int ring[2 * 64];                            // edge-crossed pixels awaiting their exact area: entry << 20 | box pixel
 };
 
 // a general triangle handed from the per-lane path to the wave queue
@@ -574,6 +575,60 @@ __device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>&
 // and the wave waits for its outstanding LDS operations
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
+// pixel (xx, yy) of entry e's box from its index (< 2^20: the float quotient is off by at most one)
+__device__ __forceinline__ void tri_queue_box_pixel(const TriQueue& q, int e, int local, int& xx, int& yy)
+{
+	const int w = q.w[e];
+	yy = int(float(local) * q.rw[e]);
+	xx = local - yy * w;
+	if (xx >= w)
+	{
+		yy++;
+		xx -= w;
+	}
+	else if (xx < 0)
+	{
+		yy--;
+		xx += w;
+	}
+}
+
+// exact areas (FP64 Green's theorem) of ring slots [start, start + count), one per lane
+template <typename MapT, bool PARITY>
+__device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q, int start, int count)
+{
+	const int lane = threadIdx.x & 63;
+	if (lane >= count) return;
+	int slot = start + lane;
+	asm volatile("" : "+v"(slot));
+	const int code = q.ring[slot & 127];
+	const int e = code >> 20, local = code & ((1 << 20) - 1);
+	int xx, yy;
+	tri_queue_box_pixel(q, e, local, xx, yy);
+	dev::Tri<double> t;
+#pragma unroll
+	for (int i = 0; i < 3; i++)
+	{
+		t.x[i] = q.x[i][e];
+		t.y[i] = q.y[i][e];
+		t.idx[i] = q.idx[i][e];
+		t.idy[i] = q.idy[i][e];
+	}
+#pragma unroll
+	for (int i = 0; i < 3; i++)
+	{
+		const int n = (i + 1) % 3;
+		t.dx[i] = t.x[n] - t.x[i];
+		t.dy[i] = t.y[n] - t.y[i];
+	}
+	const double ar = dev::tri_pixel_area<double>(t, double(xx), double(yy));
+	if (ar > 0)
+	{
+		const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx);
+		acc_flush_one<MapT, PARITY>(a, pix * 2 + ((q.hkey[e] >> 29) & 1), MapT(q.scale[e] * ar));
+	}
+}
+
 // rasterise every queued triangle: pixel k of the concatenated boxes goes to lane k mod 64
 template <typename MapT, bool PARITY>
 __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q, int qn)
@@ -639,9 +694,19 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 		total += __shfl(c, 63);
 	}
 	wave_lds_sync();
+	/*
+	Box pixels 64 at a time. A pixel strictly outside one edge's half-plane
+	has no overlap and is dropped; a pixel strictly inside all three gets the
+	whole pixel (Green's theorem returns exactly 1 there); a pixel within
+	1e-9 px of an edge, or crossed by one, goes to the ring and gets its exact
+	area in full passes of 64. rh / rt: next ring slot to process / to fill.
+	*/
+	int rh = 0, rt = 0;
 	for (int k0 = 0; k0 < total; k0 += 64)
 	{
 		const int k = k0 + lane;
+		bool crossed = false;
+		int code = 0;
 		if (k < total)
 		{
 			// the entry whose running total first exceeds k
@@ -656,43 +721,64 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 			}
 			const int e = lo;
 			const int local = k - (e > 0 ? q.end[e - 1] : 0);
-			const int w = q.w[e];
-			// local < 2^20: the float quotient is off by at most one
-			int yy = int(float(local) * q.rw[e]);
-			int xx = local - yy * w;
-			if (xx >= w)
-			{
-				yy++;
-				xx -= w;
-			}
-			else if (xx < 0)
-			{
-				yy--;
-				xx += w;
-			}
-			dev::Tri<double> t;
+			int xx, yy;
+			tri_queue_box_pixel(q, e, local, xx, yy);
+			double vx[3], vy[3];
 #pragma unroll
 			for (int i = 0; i < 3; i++)
 			{
-				t.x[i] = q.x[i][e];
-				t.y[i] = q.y[i][e];
-				t.idx[i] = q.idx[i][e];
-				t.idy[i] = q.idy[i][e];
+				vx[i] = q.x[i][e];
+				vy[i] = q.y[i][e];
 			}
+			bool outside = false, inside = true;
 #pragma unroll
 			for (int i = 0; i < 3; i++)
 			{
 				const int n = (i + 1) % 3;
-				t.dx[i] = t.x[n] - t.x[i];
-				t.dy[i] = t.y[n] - t.y[i];
+				const double dx = vx[n] - vx[i], dy = vy[n] - vy[i];
+				// E(p) = dx (p_y - y_i) - dy (p_x - x_i) > 0 inside (counter-clockwise);
+				// its largest and smallest values over the pixel's corners
+				const double ex = -dy * ((dy > 0 ? double(xx) : double(xx + 1)) - vx[i]);
+				const double ey = dx * ((dx > 0 ? double(yy + 1) : double(yy)) - vy[i]);
+				const double fx = -dy * ((dy > 0 ? double(xx + 1) : double(xx)) - vx[i]);
+				const double fy = dx * ((dx > 0 ? double(yy) : double(yy + 1)) - vy[i]);
+				const double tol = 1e-9 * (fabs(dx) + fabs(dy));
+				outside = outside || (ex + ey < -tol);
+				inside = inside && (fx + fy > tol);
 			}
-			const double ar = dev::tri_pixel_area<double>(t, double(xx), double(yy));
-			if (ar > 0)
+			if (!outside)
 			{
-				const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx);
-				acc_flush_one<MapT, PARITY>(a, pix * 2 + ((q.hkey[e] >> 29) & 1), MapT(q.scale[e] * ar));
+				if (inside)
+				{
+					const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx);
+					acc_flush_one<MapT, PARITY>(a, pix * 2 + ((q.hkey[e] >> 29) & 1), MapT(q.scale[e]));
+				}
+				else
+				{
+					crossed = true;
+					code = (e << 20) | local;
+				}
 			}
 		}
+		const unsigned long long m = __ballot(crossed);
+		if (crossed)
+		{
+			int pos = rt + int(__builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u)));
+			asm volatile("" : "+v"(pos));
+			q.ring[pos & 127] = code;
+		}
+		rt += __popcll(m);
+		if (rt - rh >= 64)
+		{
+			wave_lds_sync();
+			tri_queue_exact<MapT, PARITY>(a, q, rh, 64);
+			rh += 64;
+		}
+	}
+	if (rt > rh)
+	{
+		wave_lds_sync();
+		tri_queue_exact<MapT, PARITY>(a, q, rh, rt - rh);
 	}
 	// needles: one column strip per lane, pixels of the strip found from the
 	// triangle's y-range inside it
