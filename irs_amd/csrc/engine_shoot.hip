@@ -445,14 +445,14 @@ __device__ __forceinline__ void acc_flush(const ShootArgs& a, PixelAcc<MapT>& c)
 General triangles (crossed by more than one pixel line) are rasterised by the
 whole wave. A lane that meets one writes it, set up for per-pixel areas, into
 its wave's queue in LDS; once the queue holds more than IPM_QFLUSH triangles
-the wave walks the concatenated bounding boxes of all queued triangles, one
-pixel per lane. A lane holding the 100-pixel image of a demagnified region
-near a star then no longer stalls the other 63 (those images are a tenth of
-the general triangles and four fifths of their pixels).
+the wave walks the pixel rows of all queued triangles, one row per lane, and
+then the pixels of those rows, one per lane. A lane holding the 100-pixel
+image of a demagnified region near a star then no longer stalls the other 63
+(those images are a tenth of the general triangles and four fifths of their
+pixels).
 */
 constexpr int IPM_QCAP = 80;                 // entries per wave
 constexpr int IPM_QFLUSH = IPM_QCAP - 64;    // an enqueue adds at most 64
-constexpr int IPM_QMAX_PIX = 1 << 20;        // larger boxes are walked as needles
 
 struct TriQueue
 {
@@ -461,10 +461,10 @@ struct TriQueue
 	double scale[IPM_QCAP];                      // factor / triangle area
 	int X0[IPM_QCAP], Y0[IPM_QCAP];
 	int w[IPM_QCAP];                             // box width in pixels
-	int hkey[IPM_QCAP];                          // box height | map bit << 29 | needle << 30
-	int end[IPM_QCAP];                           // box pixels, then running total
-	float rw[IPM_QCAP];                          // 1 / w
-	int ring[2 * 64];                            // edge-crossed pixels awaiting their exact area: entry << 20 | box pixel
+	int bit[IPM_QCAP];                           // map: 0 = total or minima, 1 = saddles
+	int end[IPM_QCAP];                           // box rows, then running total
+	int ring_ey[2 * 64];                         // edge-crossed pixels awaiting their exact area: entry | row << 7
+	int ring_x[2 * 64];                          //   and their column
 };
 
 // a general triangle handed from the per-lane path to the wave queue
@@ -472,7 +472,6 @@ struct GenTri
 {
 	double x[3], y[3];   // vertices relative to (X0, Y0), in cell order
 	int X0, Y0, w, h, bit;
-	bool needle;         // box far larger than the triangle: walked by column strips
 };
 
 /*
@@ -558,11 +557,8 @@ __device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>&
 	g.y[1] = p1y - Y0;
 	g.x[2] = p2x - X0;
 	g.y[2] = p2y - Y0;
-	const double area2 = fabs((g.x[1] - g.x[0]) * (g.y[2] - g.y[0]) - (g.x[2] - g.x[0]) * (g.y[1] - g.y[0]));
+	const double area2 = (g.x[1] - g.x[0]) * (g.y[2] - g.y[0]) - (g.x[2] - g.x[0]) * (g.y[1] - g.y[0]);
 	if (area2 == 0) return false;
-	// needles: a box much larger than the triangle is walked by column strips
-	const long long npix = (long long)w * h;
-	g.needle = npix > IPM_QMAX_PIX || double(npix) > 2.0 * area2 + 4.0 * double(w + h) + 64.0;
 	g.X0 = X0;
 	g.Y0 = Y0;
 	g.w = w;
@@ -575,22 +571,56 @@ __device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>&
 // and the wave waits for its outstanding LDS operations
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-// pixel (xx, yy) of entry e's box from its index (< 2^20: the float quotient is off by at most one)
-__device__ __forceinline__ void tri_queue_box_pixel(const TriQueue& q, int e, int local, int& xx, int& yy)
+/*
+Columns of row yy of queued triangle e: [xl, xr] are the pixels that lie
+outside no edge's half-plane by more than 1e-9 of the edge's length, [il, ir]
+those strictly inside all three (Green's theorem gives exactly 1 there).
+With E(p) = dx (p_y - y_i) - dy (p_x - x_i), positive inside a
+counter-clockwise triangle, a pixel is outside edge i when E's largest value
+over its corners is below -tol and inside when its smallest is above tol.
+Over a row, dx p_y + dy x_i - dx y_i ranges over [A, B]; the remaining
+term -dy p_x bounds the column from the left (dy < 0) or the right (dy > 0),
+and a horizontal edge admits the whole row or none of it.
+*/
+__device__ __forceinline__ void tri_queue_row(const TriQueue& q, int e, int yy, int& xl, int& xr, int& il, int& ir)
 {
-	const int w = q.w[e];
-	yy = int(float(local) * q.rw[e]);
-	xx = local - yy * w;
-	if (xx >= w)
+	const double wmax = double(q.w[e] - 1);
+	const double Y = double(yy);
+	double lo = 0.0, hi = wmax, ilo = 0.0, ihi = wmax;
+#pragma unroll
+	for (int i = 0; i < 3; i++)
 	{
-		yy++;
-		xx -= w;
+		const int n = (i + 1) % 3;
+		const double xi = q.x[i][e], yi = q.y[i][e];
+		const double dx = q.x[n][e] - xi, dy = q.y[n][e] - yi, idy = q.idy[i][e];
+		const double tol = 1e-9 * (fabs(dx) + fabs(dy));
+		const double base = dy * xi - dx * yi;
+		const double A = fma(dx, dx > 0 ? Y : Y + 1.0, base);
+		const double B = fma(dx, dx > 0 ? Y + 1.0 : Y, base);
+		const double bA = (A - tol) * idy, bB = (B + tol) * idy;
+		if (dy < 0)
+		{
+			// inside: A - dy X > tol, X > bA; not outside: B - dy (X + 1) >= -tol, X >= bB - 1
+			ilo = fmax(ilo, floor(bA) + 1.0);
+			lo = fmax(lo, ceil(bB) - 1.0);
+		}
+		else if (dy > 0)
+		{
+			// inside: A - dy (X + 1) > tol, X < bA - 1; not outside: B - dy X >= -tol, X <= bB
+			ihi = fmin(ihi, ceil(bA) - 2.0);
+			hi = fmin(hi, floor(bB));
+		}
+		else
+		{
+			if (!(A > tol)) ihi = -1.0;
+			if (B < -tol) hi = -1.0;
+		}
 	}
-	else if (xx < 0)
-	{
-		yy--;
-		xx += w;
-	}
+	// lo, ilo >= 0 and hi, ihi <= w - 1 already; bound the other sides before converting
+	xl = int(fmin(lo, wmax + 1.0));
+	xr = int(fmax(hi, -1.0));
+	il = int(fmin(ilo, wmax + 1.0));
+	ir = int(fmax(ihi, -1.0));
 }
 
 // exact areas (FP64 Green's theorem) of ring slots [start, start + count), one per lane
@@ -601,10 +631,8 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	if (lane >= count) return;
 	int slot = start + lane;
 	asm volatile("" : "+v"(slot));
-	const int code = q.ring[slot & 127];
-	const int e = code >> 20, local = code & ((1 << 20) - 1);
-	int xx, yy;
-	tri_queue_box_pixel(q, e, local, xx, yy);
+	const int ey = q.ring_ey[slot & 127], xx = q.ring_x[slot & 127];
+	const int e = ey & 127, yy = ey >> 7;
 	dev::Tri<double> t;
 #pragma unroll
 	for (int i = 0; i < 3; i++)
@@ -625,11 +653,31 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	if (ar > 0)
 	{
 		const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx);
-		acc_flush_one<MapT, PARITY>(a, pix * 2 + ((q.hkey[e] >> 29) & 1), MapT(q.scale[e] * ar));
+		acc_flush_one<MapT, PARITY>(a, pix * 2 + q.bit[e], MapT(q.scale[e] * ar));
 	}
 }
 
-// rasterise every queued triangle: pixel k of the concatenated boxes goes to lane k mod 64
+// inclusive prefix sum of v over the wave; ln is the lane index
+__device__ __forceinline__ int wave_prefix_sum(int v, int ln)
+{
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1)
+	{
+		const int u = __shfl(v, ln >= d ? ln - d : ln);
+		if (ln >= d) v += u;
+	}
+	return v;
+}
+
+/*
+Rasterise every queued triangle. Rows of the concatenated boxes go 64 at a
+time to the lanes, which find their row's candidate and fully covered
+columns; the candidate pixels of those 64 rows then go to the lanes, 64 at a
+time, each finding its row as the last lane whose running pixel count does
+not exceed its own index. A fully covered pixel gets the triangle's whole
+per-pixel weight; a crossed one goes to the ring and gets its exact area in
+full passes of 64. rh / rt: next ring slot to process / to fill.
+*/
 template <typename MapT, bool PARITY>
 __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q, int qn)
 {
@@ -674,160 +722,95 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 			q.scale[e] = factor * dev::recip<double>(area2 * 0.5);
 		}
 	}
+	// source lanes from an opaque lane index, so that their addresses are
+	// formed here and not held through the shooting loop
+	int ln = lane;
+	asm volatile("" : "+v"(ln));
 	int total = 0;
 	for (int base = 0; base < qn; base += 64)
 	{
 		int e = base + lane;
 		asm volatile("" : "+v"(e));
-		int c = e < qn ? q.end[e] : 0;
-		// source lanes from an opaque lane index, so that their addresses are
-		// formed here and not held through the shooting loop
-		int ln = lane;
-		asm volatile("" : "+v"(ln));
-#pragma unroll
-		for (int d = 1; d < 64; d <<= 1)
-		{
-			const int u = __shfl(c, ln >= d ? ln - d : ln);
-			if (ln >= d) c += u;
-		}
+		const int c = wave_prefix_sum(e < qn ? q.end[e] : 0, ln);
 		if (e < qn) q.end[e] = total + c;
 		total += __shfl(c, 63);
 	}
 	wave_lds_sync();
-	/*
-	Box pixels 64 at a time. A pixel strictly outside one edge's half-plane
-	has no overlap and is dropped; a pixel strictly inside all three gets the
-	whole pixel (Green's theorem returns exactly 1 there); a pixel within
-	1e-9 px of an edge, or crossed by one, goes to the ring and gets its exact
-	area in full passes of 64. rh / rt: next ring slot to process / to fill.
-	*/
 	int rh = 0, rt = 0;
-	for (int k0 = 0; k0 < total; k0 += 64)
+	for (int r0 = 0; r0 < total; r0 += 64)
 	{
-		const int k = k0 + lane;
-		bool crossed = false;
-		int code = 0;
-		if (k < total)
+		int r = r0 + lane;
+		asm volatile("" : "+v"(r));
+		int xl = 0, cnt = 0, il = 0, ir = -1, ey = 0;
+		if (r < total)
 		{
-			// the entry whose running total first exceeds k
+			// the entry whose running row total first exceeds r
 			int lo = 0, hi = qn - 1;
 			while (lo < hi)
 			{
 				const int mid = (lo + hi) >> 1;
-				if (q.end[mid] > k)
+				if (q.end[mid] > r)
 					hi = mid;
 				else
 					lo = mid + 1;
 			}
-			const int e = lo;
-			const int local = k - (e > 0 ? q.end[e - 1] : 0);
-			int xx, yy;
-			tri_queue_box_pixel(q, e, local, xx, yy);
-			double vx[3], vy[3];
+			const int yy = r - (lo > 0 ? q.end[lo - 1] : 0);
+			int xr;
+			tri_queue_row(q, lo, yy, xl, xr, il, ir);
+			cnt = max(xr - xl + 1, 0);
+			ey = lo | (yy << 7);
+		}
+		int off = wave_prefix_sum(cnt, ln);
+		const int tot = __shfl(off, 63);
+		off -= cnt;
+		for (int k0 = 0; k0 < tot; k0 += 64)
+		{
+			const int k = min(k0 + lane, tot - 1);
+			int j = 0;
 #pragma unroll
-			for (int i = 0; i < 3; i++)
+			for (int s = 32; s > 0; s >>= 1)
 			{
-				vx[i] = q.x[i][e];
-				vy[i] = q.y[i][e];
+				const int v = __shfl(off, j + s);
+				if (v <= k) j += s;
 			}
-			bool outside = false, inside = true;
-#pragma unroll
-			for (int i = 0; i < 3; i++)
+			const int x = __shfl(xl, j) + (k - __shfl(off, j));
+			const int rey = __shfl(ey, j);
+			const int ril = __shfl(il, j), rir = __shfl(ir, j);
+			bool crossed = false;
+			if (k0 + lane < tot)
 			{
-				const int n = (i + 1) % 3;
-				const double dx = vx[n] - vx[i], dy = vy[n] - vy[i];
-				// E(p) = dx (p_y - y_i) - dy (p_x - x_i) > 0 inside (counter-clockwise);
-				// its largest and smallest values over the pixel's corners
-				const double ex = -dy * ((dy > 0 ? double(xx) : double(xx + 1)) - vx[i]);
-				const double ey = dx * ((dx > 0 ? double(yy + 1) : double(yy)) - vy[i]);
-				const double fx = -dy * ((dy > 0 ? double(xx + 1) : double(xx)) - vx[i]);
-				const double fy = dx * ((dx > 0 ? double(yy) : double(yy + 1)) - vy[i]);
-				const double tol = 1e-9 * (fabs(dx) + fabs(dy));
-				outside = outside || (ex + ey < -tol);
-				inside = inside && (fx + fy > tol);
-			}
-			if (!outside)
-			{
-				if (inside)
+				if (x >= ril && x <= rir)
 				{
-					const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx);
-					acc_flush_one<MapT, PARITY>(a, pix * 2 + ((q.hkey[e] >> 29) & 1), MapT(q.scale[e]));
+					const int e = rey & 127, yy = rey >> 7;
+					const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + x);
+					acc_flush_one<MapT, PARITY>(a, pix * 2 + q.bit[e], MapT(q.scale[e]));
 				}
 				else
 				{
 					crossed = true;
-					code = (e << 20) | local;
 				}
 			}
-		}
-		const unsigned long long m = __ballot(crossed);
-		if (crossed)
-		{
-			int pos = rt + int(__builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u)));
-			asm volatile("" : "+v"(pos));
-			q.ring[pos & 127] = code;
-		}
-		rt += __popcll(m);
-		if (rt - rh >= 64)
-		{
-			wave_lds_sync();
-			tri_queue_exact<MapT, PARITY>(a, q, rh, 64);
-			rh += 64;
+			const unsigned long long m = __ballot(crossed);
+			if (crossed)
+			{
+				int pos = rt + int(__builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u)));
+				asm volatile("" : "+v"(pos));
+				q.ring_ey[pos & 127] = rey;
+				q.ring_x[pos & 127] = x;
+			}
+			rt += __popcll(m);
+			if (rt - rh >= 64)
+			{
+				wave_lds_sync();
+				tri_queue_exact<MapT, PARITY>(a, q, rh, 64);
+				rh += 64;
+			}
 		}
 	}
 	if (rt > rh)
 	{
 		wave_lds_sync();
 		tri_queue_exact<MapT, PARITY>(a, q, rh, rt - rh);
-	}
-	// needles: one column strip per lane, pixels of the strip found from the
-	// triangle's y-range inside it
-	for (int base = 0; base < qn; base += 64)
-	{
-		int e0 = base + lane;
-		asm volatile("" : "+v"(e0));
-		unsigned long long needles = __ballot(e0 < qn && ((q.hkey[e0] >> 30) & 1));
-		while (needles)
-		{
-			const int e = base + __ffsll((long long)needles) - 1;
-			needles &= needles - 1;
-			dev::Tri<double> t;
-#pragma unroll
-			for (int i = 0; i < 3; i++)
-			{
-				t.x[i] = q.x[i][e];
-				t.y[i] = q.y[i][e];
-				t.idx[i] = q.idx[i][e];
-				t.idy[i] = q.idy[i][e];
-			}
-#pragma unroll
-			for (int i = 0; i < 3; i++)
-			{
-				const int n = (i + 1) % 3;
-				t.dx[i] = t.x[n] - t.x[i];
-				t.dy[i] = t.y[n] - t.y[i];
-			}
-			const int w = q.w[e], h = q.hkey[e] & ((1 << 29) - 1), X0 = q.X0[e], Y0 = q.Y0[e], bit = (q.hkey[e] >> 29) & 1;
-			const double scale = q.scale[e];
-			for (int c = lane; c < w; c += 64)
-			{
-				double ylo, yhi;
-				dev::tri_strip_yrange<double>(t, double(c), double(c + 1), ylo, yhi);
-				if (!(ylo <= yhi)) continue;
-				ylo = ylo > -1.0 ? ylo : -1.0;
-				yhi = yhi < double(h) ? yhi : double(h);
-				const int ya = max(0, int(floor(ylo))), yb = min(h - 1, int(floor(yhi)));
-				for (int yy = ya; yy <= yb; yy++)
-				{
-					const double ar = dev::tri_pixel_area<double>(t, double(c), double(yy));
-					if (ar > 0)
-					{
-						const long long pix = (long long)a.n1 * (Y0 + yy) + (X0 + c);
-						acc_flush_one<MapT, PARITY>(a, pix * 2 + bit, MapT(scale * ar));
-					}
-				}
-			}
-		}
 	}
 	wave_lds_sync();
 }
@@ -852,9 +835,8 @@ __device__ __forceinline__ void tri_queue_push(TriQueue& q, int& qn, bool gen, c
 		q.X0[pos] = g.X0;
 		q.Y0[pos] = g.Y0;
 		q.w[pos] = g.w;
-		q.hkey[pos] = g.h | (g.bit << 29) | (g.needle ? 1 << 30 : 0);
-		q.end[pos] = g.needle ? 0 : g.w * g.h;
-		q.rw[pos] = 1.0f / float(g.w);
+		q.bit[pos] = g.bit;
+		q.end[pos] = g.h;
 	}
 	qn += __popcll(m);
 }
@@ -1510,6 +1492,8 @@ struct ShootArgsBuilder
 		a.h2 = g.half_length_y[1];
 		a.n1 = g.num_pixels[0];
 		a.n2 = g.num_pixels[1];
+		// the triangle queue packs a box row into 24 bits
+		if (a.n2 >= (1 << 24)) throw std::runtime_error("num_pixels_y2 must be < 16777216");
 		a.s1 = double(a.n1) / (2 * a.h1);
 		a.s2 = double(a.n2) / (2 * a.h2);
 		// ipm_functions.cuh:170
