@@ -170,148 +170,36 @@ __device__ inline void d_alpha_smooth(dcx z, double ks, int rectangular, int app
 }
 
 /*
-Area of a ccw triangle (a, b, c) inside the unit pixel [0,1]^2, by Green's
-theorem: area(T n B) = closed integral of x dy over the boundary of T n B,
-which is the part of each edge of T inside B plus the part of B's right edge
-(x = 1) inside T. B's left edge has x = 0 and its horizontal edges have dy = 0,
-so they contribute nothing. Each edge is clipped to B parametrically
-(Y.-D. Liang and B. A. Barsky, ACM TOG 3(1), 1984). No polygon is stored,
-so the whole computation stays in registers.
+Area of a counter-clockwise triangle inside the pixel [X, X+1] x [Y, Y+1], by
+Green's theorem with Q(x, y) = clamp(x - X, 0, 1) on the row Y <= y <= Y + 1
+and 0 elsewhere: dQ/dx is the pixel's indicator, so the area is the sum over
+the edges of the integral of Q dy. On the part of an edge inside the row, with
+u = x - X linear in y between ua and ub, the mean of clamp(u, 0, 1) is
+  ((H - L)(H + L) / 2 + max(hi, 1) - max(lo, 1)) / (hi - lo),
+where lo, hi are ua, ub in order and L, H their values clamped to [0, 1]
+(clamp(lo, 0, 1) when lo = hi). Both ratios stay in [0, 1], so a near-vertical
+edge loses no accuracy; there are no branches.
+k[i] = dx_i / dy_i of edge i (0 for a horizontal edge, whose part inside the
+row has no height).
 */
-template <typename R>
-__device__ __forceinline__ R seg_xdy_in_box(R px, R py, R dx, R dy, R idx, R idy)
+__device__ __forceinline__ double tri_pixel_area(const double x[3], const double y[3], const double k[3], double X, double Y)
 {
-	R t0 = 0, t1 = 1;
-	if (dx != 0)
-	{
-		R ta = -px * idx, tb = (1 - px) * idx;
-		R lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
-		t0 = t0 > lo ? t0 : lo;
-		t1 = t1 < hi ? t1 : hi;
-	}
-	else if (px < 0 || px > 1 || (px == 1 && dy > 0))
-	{
-		// a vertical edge on x = 1 with the triangle on its left is the part
-		// of the pixel's right edge inside the triangle, which the x = 1 term
-		// of tri_pixel_area already counts
-		return 0;
-	}
-	if (dy != 0)
-	{
-		R ta = -py * idy, tb = (1 - py) * idy;
-		R lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
-		t0 = t0 > lo ? t0 : lo;
-		t1 = t1 < hi ? t1 : hi;
-	}
-	else if (py < 0 || py > 1)
-	{
-		return 0;
-	}
-	if (!(t0 < t1)) return 0;
-	R xa = px + t0 * dx, xb = px + t1 * dx;
-	return (xa + xb) * R(0.5) * ((t1 - t0) * dy);
-}
-
-template <typename R>
-__device__ __forceinline__ void vline_hit(R px, R py, R dx, R dy, R idx, R X, R& lo, R& hi)
-{
-	R qx = px + dx;
-	if (dx != 0)
-	{
-		if ((px - X) * (qx - X) <= 0)
-		{
-			R y = py + (X - px) * idx * dy;
-			lo = lo < y ? lo : y;
-			hi = hi > y ? hi : y;
-		}
-	}
-	else if (px == X)
-	{
-		R y2 = py + dy;
-		R l = py < y2 ? py : y2, h = py < y2 ? y2 : py;
-		lo = lo < l ? lo : l;
-		hi = hi > h ? hi : h;
-	}
-}
-
-template <typename R>
-struct Tri
-{
-	R x[3], y[3];      // ccw vertices, relative to an integer origin
-	R dx[3], dy[3];    // edge vectors v[k+1] - v[k]
-	R idx[3], idy[3];  // their reciprocals (unused when the component is zero)
-};
-
-template <typename R>
-__device__ __forceinline__ R recip(R v);
-template <>
-__device__ __forceinline__ float recip<float>(float v) { return rcp32(v); }
-template <>
-__device__ __forceinline__ double recip<double>(double v) { return rcp64(v); }
-
-template <typename R>
-__device__ __forceinline__ void tri_setup(Tri<R>& t)
-{
+	double a = 0;
 #pragma unroll
-	for (int k = 0; k < 3; k++)
+	for (int i = 0; i < 3; i++)
 	{
-		int n = (k + 1) % 3;
-		t.dx[k] = t.x[n] - t.x[k];
-		t.dy[k] = t.y[n] - t.y[k];
-		t.idx[k] = t.dx[k] != 0 ? recip<R>(t.dx[k]) : R(0);
-		t.idy[k] = t.dy[k] != 0 ? recip<R>(t.dy[k]) : R(0);
+		const int n = (i + 1) % 3;
+		const double v0 = y[i] - Y, v1 = y[n] - Y, u0 = x[i] - X;
+		const double va = fmin(fmax(v0, 0.0), 1.0), vb = fmin(fmax(v1, 0.0), 1.0);
+		const double ua = fma(va - v0, k[i], u0), ub = fma(vb - v0, k[i], u0);
+		const double lo = fmin(ua, ub), hi = fmax(ua, ub);
+		const double L = fmin(fmax(lo, 0.0), 1.0), H = fmin(fmax(hi, 0.0), 1.0);
+		const double d = hi - lo;
+		const double num = fma((H - L) * 0.5, H + L, fmax(hi, 1.0) - fmax(lo, 1.0));
+		const double mean = d > 0 ? num * rcp64(d) : L;
+		a = fma(vb - va, mean, a);
 	}
-}
-
-// area of the triangle inside pixel [X, X+1] x [Y, Y+1] (X, Y relative to the triangle origin)
-template <typename R>
-__device__ __forceinline__ R tri_pixel_area(const Tri<R>& t, R X, R Y)
-{
-	R a = 0;
-	// y-range of the triangle on the line x = 1, clipped to [0, 1] below, so
-	// an empty range can start as [1, 0]
-	R lo = R(1), hi = R(0);
-#pragma unroll
-	for (int k = 0; k < 3; k++)
-	{
-		R px = t.x[k] - X, py = t.y[k] - Y;
-		a += seg_xdy_in_box<R>(px, py, t.dx[k], t.dy[k], t.idx[k], t.idy[k]);
-		vline_hit<R>(px, py, t.dx[k], t.dy[k], t.idx[k], R(1), lo, hi);
-	}
-	lo = lo > 0 ? lo : R(0);
-	hi = hi < 1 ? hi : R(1);
-	if (hi > lo) a += hi - lo;
 	return a;
-}
-
-// y-range of the triangle inside the vertical strip [xl, xr]
-template <typename R>
-__device__ __forceinline__ void tri_strip_yrange(const Tri<R>& t, R xl, R xr, R& ylo, R& yhi)
-{
-	ylo = R(1e30);
-	yhi = R(-1e30);
-#pragma unroll
-	for (int k = 0; k < 3; k++)
-	{
-		R px = t.x[k], py = t.y[k], dx = t.dx[k], dy = t.dy[k];
-		R t0 = 0, t1 = 1;
-		if (dx != 0)
-		{
-			R ta = (xl - px) * t.idx[k], tb = (xr - px) * t.idx[k];
-			R lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
-			t0 = t0 > lo ? t0 : lo;
-			t1 = t1 < hi ? t1 : hi;
-		}
-		else if (px < xl || px > xr)
-		{
-			continue;
-		}
-		if (t0 > t1) continue;
-		R ya = py + t0 * dy, yb = py + t1 * dy;
-		R l = ya < yb ? ya : yb, h = ya < yb ? yb : ya;
-		ylo = ylo < l ? ylo : l;
-		yhi = yhi > h ? yhi : h;
-	}
 }
 
 }  // namespace dev

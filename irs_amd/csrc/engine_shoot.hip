@@ -457,7 +457,7 @@ constexpr int IPM_QFLUSH = IPM_QCAP - 64;    // an enqueue adds at most 64
 struct TriQueue
 {
 	double x[3][IPM_QCAP], y[3][IPM_QCAP];       // ccw vertices relative to (X0, Y0)
-	double idx[3][IPM_QCAP], idy[3][IPM_QCAP];   // reciprocal edge components
+	double idy[3][IPM_QCAP], k[3][IPM_QCAP];     // per edge: 1 / dy and dx / dy (0 when dy = 0)
 	double scale[IPM_QCAP];                      // factor / triangle area
 	int X0[IPM_QCAP], Y0[IPM_QCAP];
 	int w[IPM_QCAP];                             // box width in pixels
@@ -465,6 +465,7 @@ struct TriQueue
 	int end[IPM_QCAP];                           // box rows, then running total
 	int ring_ey[2 * 64];                         // edge-crossed pixels awaiting their exact area: entry | row << 7
 	int ring_x[2 * 64];                          //   and their column
+	int mark[64];                                // first slot of each entry / row starting in a pass
 };
 
 // a general triangle handed from the per-lane path to the wave queue
@@ -623,7 +624,15 @@ __device__ __forceinline__ void tri_queue_row(const TriQueue& q, int e, int yy, 
 	ir = int(fmax(ihi, -1.0));
 }
 
-// exact areas (FP64 Green's theorem) of ring slots [start, start + count), one per lane
+// adds v to pixel pix of the map (PARITY: bit 0 = minima, 1 = saddles)
+template <typename MapT, bool PARITY>
+__device__ __forceinline__ void map_add(const ShootArgs& a, long long pix, int bit, MapT v)
+{
+	MapT* map = reinterpret_cast<MapT*>(PARITY ? (bit ? a.msad : a.mmin) : a.map);
+	unsafeAtomicAdd(map + pix, v);
+}
+
+// exact areas (FP64, Green's theorem over the pixel's row) of ring slots [start, start + count), one per lane
 template <typename MapT, bool PARITY>
 __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q, int start, int count)
 {
@@ -633,39 +642,45 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	asm volatile("" : "+v"(slot));
 	const int ey = q.ring_ey[slot & 127], xx = q.ring_x[slot & 127];
 	const int e = ey & 127, yy = ey >> 7;
-	dev::Tri<double> t;
+	double x[3], y[3], k[3];
 #pragma unroll
 	for (int i = 0; i < 3; i++)
 	{
-		t.x[i] = q.x[i][e];
-		t.y[i] = q.y[i][e];
-		t.idx[i] = q.idx[i][e];
-		t.idy[i] = q.idy[i][e];
+		x[i] = q.x[i][e];
+		y[i] = q.y[i][e];
+		k[i] = q.k[i][e];
 	}
-#pragma unroll
-	for (int i = 0; i < 3; i++)
-	{
-		const int n = (i + 1) % 3;
-		t.dx[i] = t.x[n] - t.x[i];
-		t.dy[i] = t.y[n] - t.y[i];
-	}
-	const double ar = dev::tri_pixel_area<double>(t, double(xx), double(yy));
+	const double ar = dev::tri_pixel_area(x, y, k, double(xx), double(yy));
 	if (ar > 0)
-	{
-		const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx);
-		acc_flush_one<MapT, PARITY>(a, pix * 2 + q.bit[e], MapT(q.scale[e] * ar));
-	}
+		map_add<MapT, PARITY>(a, (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx), q.bit[e], MapT(q.scale[e] * ar));
 }
 
-// inclusive prefix sum of v over the wave; ln is the lane index
-__device__ __forceinline__ int wave_prefix_sum(int v, int ln)
+/*
+Inclusive scans over the wave: within each row of 16 lanes by shifts of 1,
+2, 4 and 8, then across rows by broadcasting lane 15 into rows 1 and 3 and
+lane 31 into rows 2 and 3. A lane whose source lies outside its row, or whose
+row is masked off, takes the identity.
+*/
+__device__ __forceinline__ int wave_prefix_sum(int v)
 {
-#pragma unroll
-	for (int d = 1; d < 64; d <<= 1)
-	{
-		const int u = __shfl(v, ln >= d ? ln - d : ln);
-		if (ln >= d) v += u;
-	}
+	v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
+	v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
+	v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+	v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
+	v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+	v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+	return v;
+}
+
+// the same for the maximum of values >= -1
+__device__ __forceinline__ int wave_max_scan(int v)
+{
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false));   // row_shr:1
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false));   // row_shr:2
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false));   // row_shr:4
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false));   // row_shr:8
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false));   // row_bcast:15
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false));   // row_bcast:31
 	return v;
 }
 
@@ -673,10 +688,13 @@ __device__ __forceinline__ int wave_prefix_sum(int v, int ln)
 Rasterise every queued triangle. Rows of the concatenated boxes go 64 at a
 time to the lanes, which find their row's candidate and fully covered
 columns; the candidate pixels of those 64 rows then go to the lanes, 64 at a
-time, each finding its row as the last lane whose running pixel count does
-not exceed its own index. A fully covered pixel gets the triangle's whole
-per-pixel weight; a crossed one goes to the ring and gets its exact area in
-full passes of 64. rh / rt: next ring slot to process / to fill.
+time. A lane finds its entry (or row) as the last one that starts at or
+before its own index: every entry (row) starting in the current pass marks
+its first slot in LDS, and a max-scan over the slots carries each mark
+forward, starting from the entry (row) the previous pass ended in. A fully
+covered pixel gets the triangle's whole per-pixel weight; a crossed one goes
+to the ring and gets its exact area in full passes of 64. rh / rt: next ring
+slot to process / to fill.
 */
 template <typename MapT, bool PARITY>
 __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q, int qn)
@@ -684,106 +702,110 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 	const double factor = a.factor;
 	const int lane = threadIdx.x & 63;
 	wave_lds_sync();
-	// set-up: counter-clockwise order, reciprocal edge components, factor / area
+	// set-up: counter-clockwise order, edge slopes, factor / area
 	for (int base = 0; base < qn; base += 64)
 	{
 		int e = base + lane;
 		asm volatile("" : "+v"(e));
 		if (e < qn)
 		{
-			dev::Tri<double> t;
+			double x[3], y[3];
 #pragma unroll
 			for (int i = 0; i < 3; i++)
 			{
-				t.x[i] = q.x[i][e];
-				t.y[i] = q.y[i][e];
+				x[i] = q.x[i][e];
+				y[i] = q.y[i][e];
 			}
-			double area2 = (t.x[1] - t.x[0]) * (t.y[2] - t.y[0]) - (t.x[2] - t.x[0]) * (t.y[1] - t.y[0]);
+			double area2 = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0]);
 			if (area2 < 0)
 			{
-				const double sx = t.x[1], sy = t.y[1];
-				t.x[1] = t.x[2];
-				t.y[1] = t.y[2];
-				t.x[2] = sx;
-				t.y[2] = sy;
+				const double sx = x[1], sy = y[1];
+				x[1] = x[2];
+				y[1] = y[2];
+				x[2] = sx;
+				y[2] = sy;
 				area2 = -area2;
-				q.x[1][e] = t.x[1];
-				q.y[1][e] = t.y[1];
-				q.x[2][e] = t.x[2];
-				q.y[2][e] = t.y[2];
+				q.x[1][e] = x[1];
+				q.y[1][e] = y[1];
+				q.x[2][e] = x[2];
+				q.y[2][e] = y[2];
 			}
-			dev::tri_setup<double>(t);
 #pragma unroll
 			for (int i = 0; i < 3; i++)
 			{
-				q.idx[i][e] = t.idx[i];
-				q.idy[i][e] = t.idy[i];
+				const int n = (i + 1) % 3;
+				const double dy = y[n] - y[i];
+				const double idy = dy != 0 ? dev::rcp64(dy) : 0.0;
+				q.idy[i][e] = idy;
+				q.k[i][e] = (x[n] - x[i]) * idy;
 			}
-			q.scale[e] = factor * dev::recip<double>(area2 * 0.5);
+			q.scale[e] = factor * dev::rcp64(area2 * 0.5);
 		}
 	}
-	// source lanes from an opaque lane index, so that their addresses are
-	// formed here and not held through the shooting loop
-	int ln = lane;
-	asm volatile("" : "+v"(ln));
 	int total = 0;
 	for (int base = 0; base < qn; base += 64)
 	{
 		int e = base + lane;
 		asm volatile("" : "+v"(e));
-		const int c = wave_prefix_sum(e < qn ? q.end[e] : 0, ln);
+		const int c = wave_prefix_sum(e < qn ? q.end[e] : 0);
 		if (e < qn) q.end[e] = total + c;
-		total += __shfl(c, 63);
+		total += __builtin_amdgcn_readlane(c, 63);
 	}
 	wave_lds_sync();
+	// first rows of this lane's entries (lane, lane + 64), or past the end
+	const int s0 = lane < qn ? (lane > 0 ? q.end[lane - 1] : 0) : total;
+	const int s1 = lane + 64 < qn ? q.end[lane + 63] : total;
 	int rh = 0, rt = 0;
+	int ecarry = 0;
 	for (int r0 = 0; r0 < total; r0 += 64)
 	{
-		int r = r0 + lane;
-		asm volatile("" : "+v"(r));
-		int xl = 0, cnt = 0, il = 0, ir = -1, ey = 0;
+		// the entry of row r0 + lane
+		q.mark[lane] = -1;
+		if (s0 >= r0 && s0 < r0 + 64) q.mark[s0 - r0] = lane;
+		if (s1 >= r0 && s1 < r0 + 64) q.mark[s1 - r0] = lane + 64;
+		wave_lds_sync();
+		int e = max(wave_max_scan(q.mark[lane]), ecarry);
+		ecarry = __builtin_amdgcn_readlane(e, 63);
+		e = min(e, qn - 1);
+		const int r = r0 + lane;
+		int xl = 0, cnt = 0, il = 0, ir = -1, ey = 0, w = 0;
 		if (r < total)
 		{
-			// the entry whose running row total first exceeds r
-			int lo = 0, hi = qn - 1;
-			while (lo < hi)
+			const int start = e > 0 ? q.end[e - 1] : 0;
+			const int yy = r - start;
+			if (yy >= 0 && yy < q.end[e] - start)
 			{
-				const int mid = (lo + hi) >> 1;
-				if (q.end[mid] > r)
-					hi = mid;
-				else
-					lo = mid + 1;
+				int xr;
+				tri_queue_row(q, e, yy, xl, xr, il, ir);
+				cnt = max(xr - xl + 1, 0);
+				ey = e | (yy << 7);
+				w = q.w[e];
 			}
-			const int yy = r - (lo > 0 ? q.end[lo - 1] : 0);
-			int xr;
-			tri_queue_row(q, lo, yy, xl, xr, il, ir);
-			cnt = max(xr - xl + 1, 0);
-			ey = lo | (yy << 7);
 		}
-		int off = wave_prefix_sum(cnt, ln);
-		const int tot = __shfl(off, 63);
+		int off = wave_prefix_sum(cnt);
+		const int tot = __builtin_amdgcn_readlane(off, 63);
 		off -= cnt;
+		const int xbase = xl - off;
+		int jcarry = 0;
 		for (int k0 = 0; k0 < tot; k0 += 64)
 		{
-			const int k = min(k0 + lane, tot - 1);
-			int j = 0;
-#pragma unroll
-			for (int s = 32; s > 0; s >>= 1)
-			{
-				const int v = __shfl(off, j + s);
-				if (v <= k) j += s;
-			}
-			const int x = __shfl(xl, j) + (k - __shfl(off, j));
+			// the row of candidate pixel k0 + lane
+			q.mark[lane] = -1;
+			if (cnt > 0 && off >= k0 && off < k0 + 64) q.mark[off - k0] = lane;
+			wave_lds_sync();
+			const int j = max(wave_max_scan(q.mark[lane]), jcarry);
+			jcarry = __builtin_amdgcn_readlane(j, 63);
+			const int k = k0 + lane;
+			const int x = __shfl(xbase, j) + k;
 			const int rey = __shfl(ey, j);
-			const int ril = __shfl(il, j), rir = __shfl(ir, j);
+			const int ril = __shfl(il, j), rir = __shfl(ir, j), rw = __shfl(w, j);
 			bool crossed = false;
-			if (k0 + lane < tot)
+			if (k < tot && x >= 0 && x < rw)
 			{
 				if (x >= ril && x <= rir)
 				{
-					const int e = rey & 127, yy = rey >> 7;
-					const long long pix = (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + x);
-					acc_flush_one<MapT, PARITY>(a, pix * 2 + q.bit[e], MapT(q.scale[e]));
+					const int ee = rey & 127, yy = rey >> 7;
+					map_add<MapT, PARITY>(a, (long long)a.n1 * (q.Y0[ee] + yy) + (q.X0[ee] + x), q.bit[ee], MapT(q.scale[ee]));
 				}
 				else
 				{
