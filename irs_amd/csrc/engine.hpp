@@ -190,6 +190,12 @@ private:
 	bool map_int_ = false;
 	bool parities_ = false;
 	DeviceBuffer<char> map_, map_min_, map_sad_;
+	// tiled accumulation maps of the fast IPM kernel (all zero between shots)
+	DeviceBuffer<char> tmap_, tmap_min_, tmap_sad_;
+	int tiles_x_ = 0;
+	size_t tmap_bytes_ = 0;
+	void ensure_tiled_maps();
+	void fold_tiled_maps();
 
 	DeviceBuffer<int> scratch_i_;
 	DeviceBuffer<double> scratch_d_;

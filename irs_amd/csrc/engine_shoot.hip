@@ -39,6 +39,7 @@ struct ShootArgs
 	// source-plane map
 	double cyx, cyy, h1, h2, s1, s2, factor;
 	int n1, n2;
+	int tiles_x;   // tiles per row of a tiled map (map_index)
 	// tree
 	const double *ncx, *ncy, *nhl;
 	const int *nchild0, *nbegin, *ncount, *nnnbr, *nnbr, *leaf_of_node;
@@ -389,6 +390,35 @@ global atomic. On a multi-XCD MI355X device-scope atomics are carried out
 beyond the per-XCD L2 caches, so every avoided atomic is a memory round trip.
 The key is pixel index * 2 + map (0 = total or minima, 1 = saddles).
 */
+/*
+Map layouts. The fast IPM kernel accumulates into a map stored in 128-byte
+tiles of 8 x 4 FP32 (4 x 4 FP64) pixels, tiles in row-major order, and the
+map is folded into the row-major map afterwards. A small triangle then spans
+one or two cache lines instead of one per pixel row, and the atomics (carried
+out beyond the per-XCD L2 caches) cost per line touched.
+*/
+template <typename MapT>
+struct MapTile
+{
+	static constexpr int W = sizeof(MapT) == 4 ? 8 : 4, H = 4;
+};
+
+template <typename MapT>
+__device__ __forceinline__ long long tiled_index(int tiles_x, int X, int Y)
+{
+	constexpr int W = MapTile<MapT>::W, H = MapTile<MapT>::H;
+	return ((long long)(Y / H) * tiles_x + (X / W)) * (W * H) + (Y % H) * W + (X % W);
+}
+
+template <typename MapT, bool TILED>
+__device__ __forceinline__ long long map_index(const ShootArgs& a, int X, int Y)
+{
+	if constexpr (TILED)
+		return tiled_index<MapT>(a.tiles_x, X, Y);
+	else
+		return (long long)a.n1 * Y + X;
+}
+
 template <typename MapT>
 struct PixelAcc
 {
@@ -486,7 +516,7 @@ edges on its own side (similar triangles); no area is ever differenced, so
 slivers stay exact. Returns true when the triangle is left in g for the
 wave's general rasteriser.
 */
-template <int PREC, bool PARITY, typename MapT>
+template <int PREC, bool PARITY, typename MapT, bool TILED>
 __device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>& acc, double p0x, double p0y, double p1x,
 	double p1y, double p2x, double p2y, GenTri& g)
 {
@@ -506,7 +536,7 @@ __device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>&
 	const MapT factor = MapT(a.factor);
 	if (inside && X0 == X1 && Y0 == Y1)
 	{
-		acc_add<MapT, PARITY>(a, acc, ((long long)a.n1 * Y0 + X0) * 2 + map_bit, factor);
+		acc_add<MapT, PARITY>(a, acc, map_index<MapT, TILED>(a, X0, Y0) * 2 + map_bit, factor);
 		return false;
 	}
 	const bool xcut = inside && X1 == X0 + 1 && Y0 == Y1;
@@ -542,8 +572,8 @@ __device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>&
 		// fractions t_b = ua / (ua - ub), t_c = ua / (ua - uc) of the edges from
 		// the lone vertex to the line; their product in one division
 		const MapT lone = MapT(double(a.factor) * ((ua * ua) / ((ua - ub) * (ua - uc))));
-		long long klow = (long long)a.n1 * Y0 + X0;
-		long long khigh = xcut ? klow + 1 : klow + a.n1;
+		long long klow = map_index<MapT, TILED>(a, X0, Y0);
+		long long khigh = xcut ? map_index<MapT, TILED>(a, X0 + 1, Y0) : map_index<MapT, TILED>(a, X0, Y0 + 1);
 		long long klone = lone_low ? klow : khigh, kother = lone_low ? khigh : klow;
 		acc_add<MapT, PARITY>(a, acc, klone * 2 + map_bit, lone);
 		acc_add<MapT, PARITY>(a, acc, kother * 2 + map_bit, factor - lone);
@@ -633,7 +663,7 @@ __device__ __forceinline__ void map_add(const ShootArgs& a, long long pix, int b
 }
 
 // exact areas (FP64, Green's theorem over the pixel's row) of ring slots [start, start + count), one per lane
-template <typename MapT, bool PARITY>
+template <typename MapT, bool PARITY, bool TILED>
 __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q, int start, int count)
 {
 	const int lane = threadIdx.x & 63;
@@ -652,7 +682,7 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	}
 	const double ar = dev::tri_pixel_area(x, y, k, double(xx), double(yy));
 	if (ar > 0)
-		map_add<MapT, PARITY>(a, (long long)a.n1 * (q.Y0[e] + yy) + (q.X0[e] + xx), q.bit[e], MapT(q.scale[e] * ar));
+		map_add<MapT, PARITY>(a, map_index<MapT, TILED>(a, q.X0[e] + xx, q.Y0[e] + yy), q.bit[e], MapT(q.scale[e] * ar));
 }
 
 /*
@@ -696,7 +726,7 @@ covered pixel gets the triangle's whole per-pixel weight; a crossed one goes
 to the ring and gets its exact area in full passes of 64. rh / rt: next ring
 slot to process / to fill.
 */
-template <typename MapT, bool PARITY>
+template <typename MapT, bool PARITY, bool TILED>
 __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q, int qn)
 {
 	const double factor = a.factor;
@@ -805,7 +835,7 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 				if (x >= ril && x <= rir)
 				{
 					const int ee = rey & 127, yy = rey >> 7;
-					map_add<MapT, PARITY>(a, (long long)a.n1 * (q.Y0[ee] + yy) + (q.X0[ee] + x), q.bit[ee], MapT(q.scale[ee]));
+					map_add<MapT, PARITY>(a, map_index<MapT, TILED>(a, q.X0[ee] + x, q.Y0[ee] + yy), q.bit[ee], MapT(q.scale[ee]));
 				}
 				else
 				{
@@ -824,7 +854,7 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 			if (rt - rh >= 64)
 			{
 				wave_lds_sync();
-				tri_queue_exact<MapT, PARITY>(a, q, rh, 64);
+				tri_queue_exact<MapT, PARITY, TILED>(a, q, rh, 64);
 				rh += 64;
 			}
 		}
@@ -832,7 +862,7 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 	if (rt > rh)
 	{
 		wave_lds_sync();
-		tri_queue_exact<MapT, PARITY>(a, q, rh, rt - rh);
+		tri_queue_exact<MapT, PARITY, TILED>(a, q, rh, rt - rh);
 	}
 	wave_lds_sync();
 }
@@ -869,7 +899,7 @@ constexpr int IPM_TILE_H = 64;
 
 // One tile: this wave's 63 cells of lattice columns tile_x * IPM_TILE_W + wave * 63 + [0, 63),
 // cell rows [J0, J1). Every lane of the wave must call it.
-template <int PREC, bool PARITY>
+template <int PREC, bool PARITY, bool TILED>
 __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long long tile_x, long long J0, long long J1)
 {
 	const int lane = threadIdx.x & 63;
@@ -909,11 +939,11 @@ __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long l
 			const double bxx = t ? brx : tx, byy = t ? bry : ty;
 			const double cx = t ? trx : bx, cy = t ? try_ : by;
 			GenTri g;
-			const bool gen = cell_ok && ipm_triangle<PREC, PARITY, MapT>(a, acc, ax, ay, bxx, byy, cx, cy, g);
+			const bool gen = cell_ok && ipm_triangle<PREC, PARITY, MapT, TILED>(a, acc, ax, ay, bxx, byy, cx, cy, g);
 			tri_queue_push(q, qn, gen, g);
 			if (qn > IPM_QFLUSH || (qn > 0 && t == 1 && J == J1))
 			{
-				tri_queue_flush<MapT, PARITY>(a, q, qn);
+				tri_queue_flush<MapT, PARITY, TILED>(a, q, qn);
 				qn = 0;
 			}
 		}
@@ -937,7 +967,7 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long l
 	if (J0 >= row1) return;
 	const long long J1 = min(J0 + IPM_TILE_H, row1);
 	__shared__ TriQueue queues[IPM_WAVES];
-	ipm_tile<PREC, PARITY>(a, queues[threadIdx.x >> 6], blockIdx.x, J0, J1);
+	ipm_tile<PREC, PARITY, true>(a, queues[threadIdx.x >> 6], blockIdx.x, J0, J1);
 }
 
 /* ---------------------------------------------------------------------------
@@ -980,7 +1010,7 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_batch(const ShootArgs* _
 		const long long ntx = (a.N1 + IPM_TILE_W - 1) / IPM_TILE_W;
 		const long long ty = local / ntx, tx = local - ty * ntx;
 		const long long J0 = ty * tile_h, J1 = min(J0 + (long long)tile_h, a.N2);
-		ipm_tile<PREC_MIXED, false>(a, queues[threadIdx.x >> 6], tx, J0, J1);
+		ipm_tile<PREC_MIXED, false, false>(a, queues[threadIdx.x >> 6], tx, J0, J1);
 	}
 }
 
@@ -1564,6 +1594,66 @@ void Engine::alloc_maps(int n1, int n2, bool ipm, Precision prec, bool parities)
 	zero_maps();
 }
 
+/*
+Tiled accumulation maps for the fast IPM kernel: 128-byte tiles of
+MapTile<MapT>::W x H pixels, ceil(n1 / W) tiles per row. Allocated zeroed on
+first use (and again when the map's shape or type changes); fold_tiled_maps
+adds them into the row-major maps and clears them, so they are zero between
+shots.
+*/
+void Engine::ensure_tiled_maps()
+{
+	const int W = map_fp32_ ? MapTile<float>::W : MapTile<double>::W, H = MapTile<float>::H;
+	const int tx = (npix_[0] + W - 1) / W;
+	const size_t bytes = size_t(tx) * size_t((npix_[1] + H - 1) / H) * 128;
+	if (tmap_.ptr && tiles_x_ == tx && tmap_bytes_ == bytes && (!parities_ || tmap_min_.ptr)) return;
+	tiles_x_ = tx;
+	tmap_bytes_ = bytes;
+	tmap_.alloc(bytes);
+	MLENS_HIP_CHECK(hipMemsetAsync(tmap_.ptr, 0, bytes, stream_));
+	tmap_min_.free();
+	tmap_sad_.free();
+	if (parities_)
+	{
+		tmap_min_.alloc(bytes);
+		tmap_sad_.alloc(bytes);
+		MLENS_HIP_CHECK(hipMemsetAsync(tmap_min_.ptr, 0, bytes, stream_));
+		MLENS_HIP_CHECK(hipMemsetAsync(tmap_sad_.ptr, 0, bytes, stream_));
+	}
+}
+
+// map[y][x] += tiled[y][x] and tiled[y][x] = 0; a block of 64 x 4 threads covers one tile row of 64 columns
+template <typename MapT>
+__global__ __launch_bounds__(256) void k_fold_tiles(MapT* __restrict__ map, MapT* __restrict__ tiled, int n1, int n2, int tiles_x)
+{
+	const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+	const int y = blockIdx.y * MapTile<MapT>::H + (threadIdx.x >> 6);
+	if (x >= n1 || y >= n2) return;
+	const long long t = tiled_index<MapT>(tiles_x, x, y);
+	const MapT v = tiled[t];
+	if (v != MapT(0))
+	{
+		map[(long long)n1 * y + x] += v;
+		tiled[t] = MapT(0);
+	}
+}
+
+void Engine::fold_tiled_maps()
+{
+	if (!tmap_.ptr) return;
+	dim3 grid(unsigned((npix_[0] + 63) / 64), unsigned((npix_[1] + MapTile<float>::H - 1) / MapTile<float>::H));
+	for (int k = 0; k < (parities_ ? 3 : 1); k++)
+	{
+		char* m = k == 0 ? map_.ptr : (k == 1 ? map_min_.ptr : map_sad_.ptr);
+		char* t = k == 0 ? tmap_.ptr : (k == 1 ? tmap_min_.ptr : tmap_sad_.ptr);
+		if (map_fp32_)
+			k_fold_tiles<float><<<grid, 256, 0, stream_>>>((float*)m, (float*)t, npix_[0], npix_[1], tiles_x_);
+		else
+			k_fold_tiles<double><<<grid, 256, 0, stream_>>>((double*)m, (double*)t, npix_[0], npix_[1], tiles_x_);
+	}
+	MLENS_HIP_CHECK(hipGetLastError());
+}
+
 void Engine::zero_maps()
 {
 	size_t elem = map_int_ ? sizeof(int) : (map_fp32_ ? sizeof(float) : sizeof(double));
@@ -1574,6 +1664,9 @@ void Engine::zero_maps()
 		MLENS_HIP_CHECK(hipMemsetAsync(map_min_.ptr, 0, bytes, stream_));
 		MLENS_HIP_CHECK(hipMemsetAsync(map_sad_.ptr, 0, bytes, stream_));
 	}
+	// tiled maps are zero between shots unless a shot was cut short
+	for (DeviceBuffer<char>* t : {&tmap_, &tmap_min_, &tmap_sad_})
+		if (t->ptr) MLENS_HIP_CHECK(hipMemsetAsync(t->ptr, 0, std::min(tmap_bytes_, t->count), stream_));
 }
 
 ShootStats Engine::shoot_ipm(const LensParams& lens, const GridParams& g, Precision prec, ShootKernel kern, long long row0,
@@ -1605,6 +1698,11 @@ ShootStats Engine::shoot_ipm(const LensParams& lens, const GridParams& g, Precis
 	}
 	else
 	{
+		ensure_tiled_maps();
+		a.tiles_x = tiles_x_;
+		a.map = tmap_.ptr;
+		a.mmin = parities_ ? tmap_min_.ptr : nullptr;
+		a.msad = parities_ ? tmap_sad_.ptr : nullptr;
 		dim3 grid(unsigned((a.N1 + IPM_TILE_W - 1) / IPM_TILE_W), unsigned((row1 - row0 + IPM_TILE_H - 1) / IPM_TILE_H));
 		dim3 block(64 * IPM_WAVES);
 		if (prec == PREC_FP64)
@@ -1617,6 +1715,8 @@ ShootStats Engine::shoot_ipm(const LensParams& lens, const GridParams& g, Precis
 			if (parities_) k_ipm_fast<PREC_MIXED, true><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
 			else k_ipm_fast<PREC_MIXED, false><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad);
 		}
+		MLENS_HIP_CHECK(hipGetLastError());
+		fold_tiled_maps();
 	}
 	MLENS_HIP_CHECK(hipGetLastError());
 	MLENS_HIP_CHECK(hipEventRecord(ev1_, stream_));
