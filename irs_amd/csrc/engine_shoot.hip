@@ -403,20 +403,50 @@ struct MapTile
 	static constexpr int W = sizeof(MapT) == 4 ? 8 : 4, H = 4;
 };
 
+// index of pixel (X, Y), X, Y >= 0, of a tiled map: the row's base plus the column's offset
+template <typename MapT>
+__device__ __forceinline__ long long tiled_row_base(int tiles_x, int Y)
+{
+	constexpr unsigned W = MapTile<MapT>::W, H = MapTile<MapT>::H;
+	return (long long)(unsigned(Y) / H) * tiles_x * (W * H) + (unsigned(Y) % H) * W;
+}
+
+template <typename MapT>
+__device__ __forceinline__ int tiled_col(int X)
+{
+	constexpr unsigned W = MapTile<MapT>::W, H = MapTile<MapT>::H;
+	return int((unsigned(X) / W) * (W * H) + unsigned(X) % W);
+}
+
 template <typename MapT>
 __device__ __forceinline__ long long tiled_index(int tiles_x, int X, int Y)
 {
-	constexpr int W = MapTile<MapT>::W, H = MapTile<MapT>::H;
-	return ((long long)(Y / H) * tiles_x + (X / W)) * (W * H) + (Y % H) * W + (X % W);
+	return tiled_row_base<MapT>(tiles_x, Y) + tiled_col<MapT>(X);
+}
+
+// the same for the map a kernel writes: tiled (TILED) or row-major
+template <typename MapT, bool TILED>
+__device__ __forceinline__ long long map_row_base(const ShootArgs& a, int Y)
+{
+	if constexpr (TILED)
+		return tiled_row_base<MapT>(a.tiles_x, Y);
+	else
+		return (long long)a.n1 * Y;
+}
+
+template <typename MapT, bool TILED>
+__device__ __forceinline__ int map_col(int X)
+{
+	if constexpr (TILED)
+		return tiled_col<MapT>(X);
+	else
+		return X;
 }
 
 template <typename MapT, bool TILED>
 __device__ __forceinline__ long long map_index(const ShootArgs& a, int X, int Y)
 {
-	if constexpr (TILED)
-		return tiled_index<MapT>(a.tiles_x, X, Y);
-	else
-		return (long long)a.n1 * Y + X;
+	return map_row_base<MapT, TILED>(a, Y) + map_col<MapT, TILED>(X);
 }
 
 template <typename MapT>
@@ -493,8 +523,8 @@ struct TriQueue
 	int w[IPM_QCAP];                             // box width in pixels
 	int bit[IPM_QCAP];                           // map: 0 = total or minima, 1 = saddles
 	int end[IPM_QCAP];                           // box rows, then running total
-	int ring_ey[2 * 64];                         // edge-crossed pixels awaiting their exact area: entry | row << 7
-	int ring_x[2 * 64];                          //   and their column
+	int ring_ey[2 * 64];                         // edge-crossed pixels awaiting their exact area: entry | map bit << 7 | row << 8
+	int ring_x[2 * 64];                          //   and their map column
 	int mark[64];                                // first slot of each entry / row starting in a pass
 };
 
@@ -670,8 +700,8 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	if (lane >= count) return;
 	int slot = start + lane;
 	asm volatile("" : "+v"(slot));
-	const int ey = q.ring_ey[slot & 127], xx = q.ring_x[slot & 127];
-	const int e = ey & 127, yy = ey >> 7;
+	const int ey = q.ring_ey[slot & 127], X = q.ring_x[slot & 127];
+	const int e = ey & 127, yy = ey >> 8;
 	double x[3], y[3], k[3];
 #pragma unroll
 	for (int i = 0; i < 3; i++)
@@ -680,9 +710,9 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 		y[i] = q.y[i][e];
 		k[i] = q.k[i][e];
 	}
-	const double ar = dev::tri_pixel_area(x, y, k, double(xx), double(yy));
+	const double ar = dev::tri_pixel_area(x, y, k, double(X - q.X0[e]), double(yy));
 	if (ar > 0)
-		map_add<MapT, PARITY>(a, map_index<MapT, TILED>(a, q.X0[e] + xx, q.Y0[e] + yy), q.bit[e], MapT(q.scale[e] * ar));
+		map_add<MapT, PARITY>(a, map_index<MapT, TILED>(a, X, q.Y0[e] + yy), (ey >> 7) & 1, MapT(q.scale[e] * ar));
 }
 
 /*
@@ -797,25 +827,35 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 		int e = max(wave_max_scan(q.mark[lane]), ecarry);
 		ecarry = __builtin_amdgcn_readlane(e, 63);
 		e = min(e, qn - 1);
+		// this lane's row: candidate columns from xa (absolute), cnt of them, fully
+		// covered ones [ila, ira], the map index of the row, the weight of a full pixel
 		const int r = r0 + lane;
-		int xl = 0, cnt = 0, il = 0, ir = -1, ey = 0, w = 0;
+		int xa = 0, cnt = 0, ila = 0, ira = -1, ey = 0;
+		long long rbase = 0;
+		MapT full = 0;
 		if (r < total)
 		{
 			const int start = e > 0 ? q.end[e - 1] : 0;
 			const int yy = r - start;
 			if (yy >= 0 && yy < q.end[e] - start)
 			{
-				int xr;
+				int xl, xr, il, ir;
 				tri_queue_row(q, e, yy, xl, xr, il, ir);
 				cnt = max(xr - xl + 1, 0);
-				ey = e | (yy << 7);
-				w = q.w[e];
+				const int X0 = q.X0[e];
+				xa = X0 + xl;
+				ila = X0 + il;
+				ira = X0 + ir;
+				ey = e | (q.bit[e] << 7) | (yy << 8);
+				rbase = map_row_base<MapT, TILED>(a, q.Y0[e] + yy);
+				full = MapT(q.scale[e]);
 			}
 		}
 		int off = wave_prefix_sum(cnt);
 		const int tot = __builtin_amdgcn_readlane(off, 63);
+		const int lim = off;   // candidate k of this row: off - cnt <= k < lim
 		off -= cnt;
-		const int xbase = xl - off;
+		const int xbase = xa - off;
 		int jcarry = 0;
 		for (int k0 = 0; k0 < tot; k0 += 64)
 		{
@@ -827,20 +867,17 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 			jcarry = __builtin_amdgcn_readlane(j, 63);
 			const int k = k0 + lane;
 			const int x = __shfl(xbase, j) + k;
-			const int rey = __shfl(ey, j);
-			const int ril = __shfl(il, j), rir = __shfl(ir, j), rw = __shfl(w, j);
+			const int rlim = __shfl(lim, j), rey = __shfl(ey, j);
+			const int rila = __shfl(ila, j), rira = __shfl(ira, j);
+			const long long rb = __shfl(rbase, j);
+			const MapT rfull = __shfl(full, j);
 			bool crossed = false;
-			if (k < tot && x >= 0 && x < rw)
+			if (k < tot && k < rlim)
 			{
-				if (x >= ril && x <= rir)
-				{
-					const int ee = rey & 127, yy = rey >> 7;
-					map_add<MapT, PARITY>(a, map_index<MapT, TILED>(a, q.X0[ee] + x, q.Y0[ee] + yy), q.bit[ee], MapT(q.scale[ee]));
-				}
+				if (x >= rila && x <= rira)
+					map_add<MapT, PARITY>(a, rb + map_col<MapT, TILED>(x), (rey >> 7) & 1, rfull);
 				else
-				{
 					crossed = true;
-				}
 			}
 			const unsigned long long m = __ballot(crossed);
 			if (crossed)
@@ -1544,8 +1581,8 @@ struct ShootArgsBuilder
 		a.h2 = g.half_length_y[1];
 		a.n1 = g.num_pixels[0];
 		a.n2 = g.num_pixels[1];
-		// the triangle queue packs a box row into 24 bits
-		if (a.n2 >= (1 << 24)) throw std::runtime_error("num_pixels_y2 must be < 16777216");
+		// the triangle queue packs a box row into 23 bits
+		if (a.n2 >= (1 << 23)) throw std::runtime_error("num_pixels_y2 must be < 8388608");
 		a.s1 = double(a.n1) / (2 * a.h1);
 		a.s2 = double(a.n2) / (2 * a.h2);
 		// ipm_functions.cuh:170
