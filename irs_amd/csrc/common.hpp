@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <iostream>
 #include <string>
 
 namespace mlens
@@ -69,6 +70,16 @@ struct GridParams
 };
 
 }  // namespace mlens
+
+// prints `what` to stdout when verbose >= level
+#define MLENS_VERBOSE(what, verbose, level) \
+	do                                      \
+	{                                       \
+		if ((verbose) >= (level))           \
+		{                                   \
+			std::cout << what;              \
+		}                                   \
+	} while (false)
 
 #define MLENS_HIP_CHECK(expr)                                                                          \
 	do                                                                                                 \

@@ -169,6 +169,40 @@ __device__ inline void d_alpha_smooth(dcx z, double ks, int rectangular, int app
 	}
 }
 
+/* alpha_smooth.cuh:252-316: d^2 alpha_smooth / dzbar^2 */
+__device__ inline dcx d2_alpha_smooth(dcx z, double ks, int rectangular, int approx, int taylor_smooth, dcx corner)
+{
+	const double inv_pi = 0.31830988618379067154;
+	dcx zb = conj(z);
+	if (rectangular)
+	{
+		if (approx)
+		{
+			// the derivative of the even series of d alpha_smooth / dzbar, term by term
+			dcx r = cdiv(zb, corner);
+			double ph = atan2(corner.im, corner.re);
+			dcx d = {0, 0};
+			for (int i = taylor_smooth - 1; i >= 2; i -= 2)
+			{
+				dcx e = cexp(dcx{0, 2 * ph * i});
+				d = add(d, dcx{1 - e.re, -e.im});
+				d = mul(d, mul(r, r));
+			}
+			d = cdiv(d, zb);
+			d = scale(d, 2);
+			return mul(d, dcx{0, -ks * inv_pi});
+		}
+		dcx c1 = sub(conj(corner), zb), c2 = sub(corner, zb);
+		dcx c3 = sub(dcx{-corner.re, -corner.im}, zb), c4 = sub(dcx{-corner.re, corner.im}, zb);
+		dcx one = {1, 0};
+		dcx d = sub(add(add(scale(cdiv(one, c1), -1), cdiv(one, c2)), cdiv(one, c3)), cdiv(one, c4));
+		return mul(d, dcx{0, -ks * inv_pi});
+	}
+	if (approx) return dcx{0, 0};
+	double cabs_ = hypot(corner.re, corner.im);
+	return cdiv(dcx{-2 * ks * cabs_ * cabs_, 0}, mul(zb, mul(zb, zb)));
+}
+
 /*
 Area of a counter-clockwise triangle inside the pixel [X, X+1] x [Y, Y+1], by
 Green's theorem with Q(x, y) = clamp(x - X, 0, 1) on the row Y <= y <= Y + 1

@@ -64,6 +64,13 @@ struct StarStats
 	double m_tot, m2_tot, m2_ln_tot, m_min, m_max;
 };
 
+struct CCFResult
+{
+	double t_init_ms = 0, t_ccs_ms = 0, t_caustics_ms = 0;   // starting roots, phase sweep, outputs (HIP events)
+	double max_error = 0;   // largest error bound on 1/mu over all roots
+	bool has_nan = false;   // some error bound is not finite
+};
+
 struct ShootStats
 {
 	double ms = 0;                 // kernel time (HIP events)
@@ -119,6 +126,14 @@ public:
 	void map_minmax(int which, double* mn, double* mx);
 	void map_histogram(int which, bool log10_of_value, int factor, long long hist_min, long long length,
 		long long* host_hist);
+
+	/* critical curves (ccf_functions.cuh, ccf.cuh): refine nroots roots per phase for
+	   nphi + nb phases (nb branches of nphi / nb + 1 phases), starting from init_roots
+	   (nb x nroots complex, the roots at each branch's centre phase). Outputs are
+	   root-major, nroots x (nphi + nb): ccs and caustics complex (re, im), mu_scales
+	   real; caustics and mu_scales may be null. Needs build_tree and fmm first. */
+	CCFResult ccf_solve(const LensParams& lens, const double* init_roots, int nroots, int nphi, int nb, bool circle_pole,
+		double* ccs, double* caustics, double* mu_scales, int init_iters = 30, int step_iters = 20);
 
 	/* thin test entry: lens equation at arbitrary points through the tree */
 	void deflect_points(const LensParams& lens, const double* z, int n, Precision prec, double* w, double* mu);

@@ -30,15 +30,6 @@
 namespace mlens
 {
 
-#define MLENS_VERBOSE(what, verbose, level) \
-	do                                      \
-	{                                       \
-		if ((verbose) >= (level))           \
-		{                                   \
-			std::cout << what;              \
-		}                                   \
-	} while (false)
-
 template <typename T, bool IS_IPM>
 class MapMaker
 {

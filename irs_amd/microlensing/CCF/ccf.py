@@ -1,0 +1,322 @@
+"""Critical curves and caustics of a star field: the CCF class of
+microlensing/CCF/ccf.py:6-361 over lib_ccf.so, with the same constructor
+arguments, property validation, run() / save() and plots."""
+
+import numpy as np
+
+from . import lib_ccf
+
+MASS_FUNCTIONS = ("equal", "uniform", "salpeter", "kroupa", "optical_depth")
+
+
+class CCF(object):
+    def __init__(self, kappa_tot: float = None, shear: float = None, smooth_fraction: float = None, kappa_star: float = None,
+                 theta_star: float = None, mass_function: str = None, m_solar: float = None, m_lower: float = None,
+                 m_upper: float = None, rectangular: bool = None, approx: bool = None, safety_scale: float = None,
+                 num_stars: int = None, starfile: str = None, num_phi: int = None, num_branches: int = None,
+                 random_seed: int = None, write_stars: bool = None, write_critical_curves: bool = None,
+                 write_caustics: bool = None, write_length_scales: bool = None, outfile_prefix: str = None,
+                 verbose: int = 0):
+        '''
+        :param kappa_tot: total convergence
+        :param shear: shear
+        :param smooth_fraction: fraction of convergence due to smoothly distributed mass
+        :param kappa_star: convergence in point mass lenses
+        :param theta_star: Einstein radius of a unit mass point lens in arbitrary units
+        :param mass_function: equal, uniform, Salpeter, Kroupa or optical_depth
+        :param m_solar: solar mass in arbitrary units
+        :param m_lower: lower mass cutoff in solar mass units
+        :param m_upper: upper mass cutoff in solar mass units
+        :param rectangular: whether the star field is rectangular (True) or circular (False)
+        :param approx: whether terms for alpha_smooth are approximated (True) or exact (False)
+        :param safety_scale: ratio of the size of the star field to the size of the shooting region
+        :param num_stars: number of stars
+        :param starfile: binary star file (num_stars, rectangular, corner, theta_star, stars), or a text file
+                         of x1 x2 m per line in units where theta_star = 1; takes precedence for all star information
+        :param num_phi: number of steps of phi in [0, 2 pi]
+        :param num_branches: number of branches of phi in [0, 2 pi]
+        :param random_seed: random seed for the star field; 0 is reserved for star input files
+        :param write_stars: whether to write stars
+        :param write_critical_curves: whether to write critical curves
+        :param write_caustics: whether to write caustics
+        :param write_length_scales: whether to write magnification length scales
+        :param outfile_prefix: prefix of the output file names
+        '''
+        self.lib = lib_ccf.lib
+        self.obj = self.lib.init()
+        self.verbose = verbose
+
+        self.kappa_tot = kappa_tot
+        self.shear = shear
+
+        if starfile is None:
+            if smooth_fraction is not None and kappa_star is None:
+                kappa_star = (1 - smooth_fraction) * self.kappa_tot
+            self.kappa_star = kappa_star
+            self.theta_star = theta_star
+            self.mass_function = mass_function
+            self.m_solar = m_solar
+            self.m_lower = m_lower
+            self.m_upper = m_upper
+            if self.m_lower > self.m_upper:
+                raise ValueError("m_lower must be <= m_upper")
+            self.num_stars = num_stars
+            self.random_seed = random_seed
+
+        self.rectangular = rectangular
+        self.approx = approx
+        self.safety_scale = safety_scale
+
+        self.starfile = starfile
+
+        self.num_phi = num_phi
+        if self.num_phi % 2 != 0:
+            raise ValueError("num_phi must be an even number")
+        self.num_branches = num_branches
+        if self.num_phi % (2 * self.num_branches) != 0:
+            raise ValueError("num_phi must be a multiple of 2 * num_branches")
+
+        self.write_stars = write_stars
+        self.write_critical_curves = write_critical_curves
+        self.write_caustics = write_caustics
+        self.write_mu_length_scales = write_length_scales
+
+        self.outfile_prefix = outfile_prefix
+
+    def close(self):
+        if self.obj:
+            self.lib.ccf_free(self.obj)
+            self.obj = None
+
+    def _num(self, name, value, minimum=None, maximum=None, cast=float):
+        if value is None:
+            return
+        if minimum is not None and value < minimum:
+            raise ValueError(f"{name} must be >= {minimum}")
+        if maximum is not None and value > maximum:
+            raise ValueError(f"{name} must be <= {maximum}")
+        getattr(self.lib, f"set_{name}")(self.obj, cast(value))
+
+    @property
+    def kappa_tot(self):
+        return self.lib.get_kappa_tot(self.obj)
+
+    @kappa_tot.setter
+    def kappa_tot(self, value):
+        self._num("kappa_tot", value, 0)
+
+    @property
+    def shear(self):
+        return self.lib.get_shear(self.obj)
+
+    @shear.setter
+    def shear(self, value):
+        self._num("shear", value)
+
+    @property
+    def mu_ave(self):
+        return 1 / ((1 - self.kappa_tot) ** 2 - self.shear ** 2)
+
+    @property
+    def kappa_star(self):
+        return self.lib.get_kappa_star(self.obj)
+
+    @kappa_star.setter
+    def kappa_star(self, value):
+        if value is not None:
+            if value < 0:
+                raise ValueError("kappa_star must be >= 0")
+            if value > self.kappa_tot:
+                raise ValueError("kappa_star must be <= kappa_tot")
+            self.lib.set_kappa_star(self.obj, float(value))
+
+    @property
+    def stellar_fraction(self):
+        return self.kappa_star / self.kappa_tot
+
+    @property
+    def smooth_fraction(self):
+        return 1 - self.kappa_star / self.kappa_tot
+
+    @property
+    def theta_star(self):
+        return self.lib.get_theta_star(self.obj)
+
+    @theta_star.setter
+    def theta_star(self, value):
+        self._num("theta_star", value, 0)
+
+    @property
+    def mass_function(self):
+        return self.lib.get_mass_function(self.obj).decode("utf-8")
+
+    @mass_function.setter
+    def mass_function(self, value):
+        if value is not None:
+            if value.lower() not in MASS_FUNCTIONS:
+                raise ValueError("mass_function must be equal, uniform, Salpeter, Kroupa, or optical_depth")
+            self.lib.set_mass_function(self.obj, value.lower().encode("utf-8"))
+
+    @property
+    def m_solar(self):
+        return self.lib.get_m_solar(self.obj)
+
+    @m_solar.setter
+    def m_solar(self, value):
+        self._num("m_solar", value, 0)
+
+    @property
+    def m_lower(self):
+        return self.lib.get_m_lower(self.obj)
+
+    @m_lower.setter
+    def m_lower(self, value):
+        self._num("m_lower", value, 0)
+
+    @property
+    def m_upper(self):
+        return self.lib.get_m_upper(self.obj)
+
+    @m_upper.setter
+    def m_upper(self, value):
+        self._num("m_upper", value, 0)
+
+    @property
+    def rectangular(self):
+        return bool(self.lib.get_rectangular(self.obj))
+
+    @rectangular.setter
+    def rectangular(self, value):
+        if value is not None:
+            self.lib.set_rectangular(self.obj, int(value))
+
+    @property
+    def approx(self):
+        return bool(self.lib.get_approx(self.obj))
+
+    @approx.setter
+    def approx(self, value):
+        if value is not None:
+            self.lib.set_approx(self.obj, int(value))
+
+    @property
+    def safety_scale(self):
+        return self.lib.get_safety_scale(self.obj)
+
+    @safety_scale.setter
+    def safety_scale(self, value):
+        self._num("safety_scale", value, 1.1)
+
+    @property
+    def num_stars(self):
+        return self.lib.get_num_stars(self.obj)
+
+    @num_stars.setter
+    def num_stars(self, value):
+        self._num("num_stars", value, 1, cast=int)
+
+    @property
+    def starfile(self):
+        return self.lib.get_starfile(self.obj).decode("utf-8")
+
+    @starfile.setter
+    def starfile(self, value):
+        if value is not None:
+            self.lib.set_starfile(self.obj, value.encode("utf-8"))
+
+    @property
+    def num_phi(self):
+        return self.lib.get_num_phi(self.obj)
+
+    @num_phi.setter
+    def num_phi(self, value):
+        self._num("num_phi", value, 1, cast=int)
+
+    @property
+    def num_branches(self):
+        return self.lib.get_num_branches(self.obj)
+
+    @num_branches.setter
+    def num_branches(self, value):
+        self._num("num_branches", value, 1, cast=int)
+
+    @property
+    def random_seed(self):
+        return self.lib.get_random_seed(self.obj)
+
+    @random_seed.setter
+    def random_seed(self, value):
+        if value is not None:
+            self.lib.set_random_seed(self.obj, int(value))
+
+    def _flag(name):
+        def get(self):
+            return bool(getattr(self.lib, f"get_{name}")(self.obj))
+
+        def put(self, value):
+            if value is not None:
+                getattr(self.lib, f"set_{name}")(self.obj, int(value))
+        return property(get, put)
+
+    write_stars = _flag("write_stars")
+    write_critical_curves = _flag("write_critical_curves")
+    write_caustics = _flag("write_caustics")
+    write_mu_length_scales = _flag("write_mu_length_scales")
+
+    @property
+    def outfile_prefix(self):
+        return self.lib.get_outfile_prefix(self.obj).decode("utf-8")
+
+    @outfile_prefix.setter
+    def outfile_prefix(self, value):
+        if value is not None:
+            self.lib.set_outfile_prefix(self.obj, value.encode("utf-8"))
+
+    @property
+    def corner(self):
+        return (self.lib.get_corner_x1(self.obj), self.lib.get_corner_x2(self.obj))
+
+    @property
+    def num_roots(self):
+        return self.lib.get_num_roots(self.obj)
+
+    def run(self):
+        if not self.lib.run(self.obj, self.verbose):
+            raise Exception("Error running CCF")
+        shape = (self.num_roots * self.num_branches, self.num_phi // self.num_branches + 1)
+        self.critical_curves = np.ctypeslib.as_array(self.lib.get_critical_curves(self.obj), shape=shape + (2,)).copy()
+        if self.write_caustics:
+            self.caustics = np.ctypeslib.as_array(self.lib.get_caustics(self.obj), shape=shape + (2,)).copy()
+        else:
+            self.caustics = None
+        if self.write_mu_length_scales:
+            self.mu_length_scales = np.ctypeslib.as_array(self.lib.get_mu_length_scales(self.obj), shape=shape).copy()
+        else:
+            self.mu_length_scales = None
+        self.stars = np.ctypeslib.as_array(self.lib.get_stars(self.obj), shape=(self.num_stars, 3)).copy()
+
+    @property
+    def t_ccs(self):
+        return self.lib.get_t_ccs(self.obj)
+
+    @property
+    def t_init_roots(self):
+        return self.lib.get_t_init_roots(self.obj)
+
+    @property
+    def max_error(self):
+        return self.lib.get_max_error(self.obj)
+
+    def save(self):
+        if not self.lib.save(self.obj, self.verbose):
+            raise Exception("Error saving CCF")
+
+    def plot_critical_curves(self, fig, ax, **kwargs):
+        ax.plot(self.critical_curves[:, :, 0].T, self.critical_curves[:, :, 1].T, **kwargs)
+        ax.set_xlabel('$x_1 / \\theta_★$')
+        ax.set_ylabel('$x_2 / \\theta_★$')
+
+    def plot_caustics(self, fig, ax, **kwargs):
+        ax.plot(self.caustics[:, :, 0].T, self.caustics[:, :, 1].T, **kwargs)
+        ax.set_xlabel('$y_1 / \\theta_★$')
+        ax.set_ylabel('$y_2 / \\theta_★$')
