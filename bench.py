@@ -224,6 +224,49 @@ def surrogate_bench(n_train=20000, n_cal=1024, n_test=2048, epochs=150, seed=11)
     return out
 
 
+def ccf_bench(num_stars=137, num_phi=100, seed=3, runs=3):
+    """Critical curves and caustics at the reference's default configuration
+    (ccf.cuh:42-62: kappa 0.3, gamma 0.3, kappa_* 0.27, equal masses, 137
+    stars, 100 phases, one branch): GPU phase-sweep time (HIP events), root
+    updates per second, and the FP64 oracle (direct sums, numpy) on the same
+    star field on the host."""
+    from irs_amd.microlensing.CCF.ccf import CCF
+    from oracle import ccf as OC
+    from oracle import oracle as O
+    kw = dict(kappa_tot=0.3, shear=0.3, kappa_star=0.27, num_stars=num_stars, num_phi=num_phi, num_branches=1,
+              random_seed=seed, write_stars=False, write_length_scales=True)
+    t_ccs, t_init = [], []
+    for _ in range(runs + 1):
+        c = CCF(**kw)
+        c.run()
+        t_ccs.append(c.t_ccs)
+        t_init.append(c.t_init_roots)
+        res = (c.num_roots, c.max_error, c.critical_curves, c.corner)
+        c.close()
+    nroots, max_err, cc, corner = res
+    t_ccs, t_init = float(np.median(t_ccs[1:])), float(np.median(t_init[1:]))
+    sweep_updates = nroots * 2 * (num_phi // 2) * 20
+    out = {"num_stars": num_stars, "num_roots": nroots, "num_phi": num_phi, "t_init_roots_s": t_init, "t_ccs_s": t_ccs,
+           "root_updates_per_s": sweep_updates / t_ccs, "max_error_inv_mu": max_err}
+    # the oracle on the same field (equal masses: the generation corner is the final corner)
+    q = np.array([1 / abs(1 - 0.3 + 0.3), 1 / abs(1 - 0.3 - 0.3)])
+    gen_corner = q / np.hypot(*q) * np.sqrt(num_stars / 0.27)
+    if not os.path.exists(O.LIB_PATH):
+        O.build()
+    stars = O.generate_stars(num_stars, 0, tuple(gen_corner), "equal", 1.0, 1.0, 1.0, seed)
+    prob = OC.Problem(stars, 0.3, 0.3, 1.0, 0.27, corner)
+    t0 = time.perf_counter()
+    roots, _, err = OC.solve(prob, num_phi, 1)
+    t_cpu = time.perf_counter() - t0
+    z = cc[..., 0] + 1j * cc[..., 1]
+    worst = max(float(np.abs(z[:, k, None] - roots[None, :, k]).min(1).max()) for k in range(z.shape[1]))
+    out.update({"cpu_oracle_s": t_cpu, "cpu_oracle_threads": 1, "cpu_kind": "port",
+                "speedup_sweep_vs_cpu": t_cpu / (t_ccs + t_init), "max_root_distance_vs_oracle": worst,
+                "oracle_max_error_inv_mu": err})
+    log(f"[ccf] {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +283,8 @@ def main():
                     help="instead of the map benchmark: this many heterogeneous maps in one graph launch")
     ap.add_argument("--surrogate", action="store_true",
                     help="instead of the map benchmark: train and evaluate the histogram surrogate")
+    ap.add_argument("--ccf", action="store_true",
+                    help="instead of the map benchmark: critical curves and caustics at the reference's defaults")
     ap.add_argument("--m2l-stress", action="store_true",
                     help="instead of the map benchmark: MFMA M2L against FP64 M2L on the 20000^2 FMM stress workload")
     args = ap.parse_args()
@@ -250,6 +295,9 @@ def main():
 
     if args.m2l_stress:
         print(json.dumps({"m2l_stress": m2l_stress((10, 20, 30))}), flush=True)
+        return
+    if args.ccf:
+        print(json.dumps({"ccf": ccf_bench()}), flush=True)
         return
     if args.batch:
         print(json.dumps({"batch": batch_bench(args.batch)}), flush=True)

@@ -1,12 +1,17 @@
-// Command-line map maker: the options, defaults, messages and exit codes of
-// the reference's executables (src/main_ipm_irs.cu:41-606), built as
-// irs_amd/bin/ipm (float), ipm_double and irs (float) over this engine.
+// Command-line programs: the options, defaults, messages and exit codes of
+// the reference's executables (src/main_ipm_irs.cu:41-606, src/main_ccf.cu:29-523),
+// built as irs_amd/bin/ipm (float), ipm_double and irs (float) over the map
+// engine, and irs_amd/bin/ccf (double) over the critical-curve solver.
 //
 // Every option takes a value: `ipm -k 0.45 -y 0.5 -s 0.2 -mf kroupa ...`.
-// Star-field options (-ks, -s, -t, -mf, -ms, -ml, -mh, -r, -rs) are ignored
-// when a star file is given; -s is ignored when -ks is given and otherwise
-// sets kappa_star = (1 - s) kappa_tot after all options are read.
+// Star-field options are ignored when a star file is given; -s is ignored
+// when -ks is given and otherwise sets kappa_star = (1 - s) kappa_tot after
+// all options are read.
+#ifdef MLENS_CLI_CCF
+#include "ccf_maker.hpp"
+#else
 #include "map_maker.hpp"
+#endif
 
 #include <algorithm>
 #include <cctype>
@@ -16,15 +21,17 @@
 #include <string>
 #include <vector>
 
-#if !defined(MLENS_CLI_IPM) && !defined(MLENS_CLI_IRS)
-#error "define MLENS_CLI_IPM or MLENS_CLI_IRS"
+#if !defined(MLENS_CLI_IPM) && !defined(MLENS_CLI_IRS) && !defined(MLENS_CLI_CCF)
+#error "define MLENS_CLI_IPM, MLENS_CLI_IRS or MLENS_CLI_CCF"
 #endif
 #ifndef MLENS_CLI_DTYPE
 #define MLENS_CLI_DTYPE float
 #endif
 
 using dtype = MLENS_CLI_DTYPE;
-#ifdef MLENS_CLI_IPM
+#if defined(MLENS_CLI_CCF)
+using Maker = mlens::CCFMaker<dtype>;
+#elif defined(MLENS_CLI_IPM)
 using Maker = mlens::MapMaker<dtype, true>;
 #else
 using Maker = mlens::MapMaker<dtype, false>;
@@ -82,6 +89,61 @@ Option opt(const char* s, const char* l, const char* name, Kind k, bool star, F&
 	return Option{s, l, name, k, star, std::forward<F>(set), std::move(get)};
 }
 
+#ifdef MLENS_CLI_CCF
+// main_ccf.cu:29-55
+std::vector<Option> options()
+{
+	Maker& m = maker;
+	return {
+		opt("-k", "--kappa_tot", "kappa_tot", REAL, false, [&](double d, int, const std::string&) { m.kappa_tot = dtype(d); },
+			[&] { return show(m.kappa_tot); }),
+		opt("-y", "--shear", "shear", REAL, false, [&](double d, int, const std::string&) { m.shear = dtype(d); },
+			[&] { return show(m.shear); }),
+		opt("-s", "--smooth_fraction", "smooth_fraction", REAL, true,
+			[&](double d, int, const std::string&) { smooth_fraction = dtype(d); }, [&] { return show(smooth_fraction); }),
+		opt("-ks", "--kappa_star", "kappa_star", REAL, true, [&](double d, int, const std::string&) { m.kappa_star = dtype(d); },
+			[&] { return show(m.kappa_star); }),
+		opt("-t", "--theta_star", "theta_star", REAL, true, [&](double d, int, const std::string&) { m.theta_star = dtype(d); },
+			[&] { return show(m.theta_star); }),
+		opt("-mf", "--mass_function", "mass_function", TEXT, true,
+			[&](double, int, const std::string& s) { m.mass_function_str = lowercase(s); }, [&] { return m.mass_function_str; }),
+		opt("-ms", "--m_solar", "m_solar", REAL, false, [&](double d, int, const std::string&) { m.m_solar = dtype(d); },
+			[&] { return show(m.m_solar); }),
+		opt("-ml", "--m_lower", "m_lower", REAL, true, [&](double d, int, const std::string&) { m.m_lower = dtype(d); },
+			[&] { return show(m.m_lower); }),
+		opt("-mh", "--m_upper", "m_upper", REAL, true, [&](double d, int, const std::string&) { m.m_upper = dtype(d); },
+			[&] { return show(m.m_upper); }),
+		opt("-r", "--rectangular", "rectangular", INTEGER, true, [&](double, int i, const std::string&) { m.rectangular = i; },
+			[&] { return show(m.rectangular); }),
+		opt("-a", "--approx", "approx", INTEGER, false, [&](double, int i, const std::string&) { m.approx = i; },
+			[&] { return show(m.approx); }),
+		opt("-ss", "--safety_scale", "safety_scale", REAL, false,
+			[&](double d, int, const std::string&) { m.safety_scale = dtype(d); }, [&] { return show(m.safety_scale); }),
+		opt("-ns", "--num_stars", "num_stars", INTEGER, true, [&](double, int i, const std::string&) { m.num_stars = i; },
+			[&] { return show(m.num_stars); }),
+		opt("-sf", "--starfile", "starfile", TEXT, false, [&](double, int, const std::string& s) { m.starfile = s; },
+			[&] { return m.starfile; }),
+		opt("-np", "--num_phi", "num_phi", INTEGER, false, [&](double, int i, const std::string&) { m.num_phi = i; },
+			[&] { return show(m.num_phi); }),
+		opt("-nb", "--num_branches", "num_branches", INTEGER, false,
+			[&](double, int i, const std::string&) { m.num_branches = i; }, [&] { return show(m.num_branches); }),
+		opt("-rs", "--random_seed", "random_seed", INTEGER, true, [&](double, int i, const std::string&) { m.random_seed = i; },
+			[&] { return show(m.random_seed); }),
+		opt("-ws", "--write_stars", "write_stars", INTEGER, false, [&](double, int i, const std::string&) { m.write_stars = i; },
+			[&] { return show(m.write_stars); }),
+		opt("-wcc", "--write_critical_curves", "write_critical_curves", INTEGER, false,
+			[&](double, int i, const std::string&) { m.write_critical_curves = i; },
+			[&] { return show(m.write_critical_curves); }),
+		opt("-wc", "--write_caustics", "write_caustics", INTEGER, false,
+			[&](double, int i, const std::string&) { m.write_caustics = i; }, [&] { return show(m.write_caustics); }),
+		opt("-wls", "--write_length_scales", "write_length_scales", INTEGER, false,
+			[&](double, int i, const std::string&) { m.write_mu_length_scales = i; },
+			[&] { return show(m.write_mu_length_scales); }),
+		opt("-o", "--outfile_prefix", "outfile_prefix", TEXT, false,
+			[&](double, int, const std::string& s) { m.outfile_prefix = s; }, [&] { return m.outfile_prefix; }),
+	};
+}
+#else
 std::vector<Option> options()
 {
 	Maker& m = maker;
@@ -150,6 +212,7 @@ std::vector<Option> options()
 			[&] { return show(m.m2l_mode); }),
 	};
 }
+#endif
 
 void usage(const char* name)
 {
@@ -163,8 +226,11 @@ void usage(const char* name)
 		std::cout << "  " << flags << o.name << ". Default value: " << o.get() << "\n";
 	}
 	std::cout << "Star-field options are ignored when a star file is given; -s is ignored\n"
-				 "when -ks is given. --precision (0 FP64, 1 mixed), --expansion_order\n"
-				 "(0 automatic) and --m2l_mode (0-3) are this engine's additions.\n";
+				 "when -ks is given.\n";
+#ifndef MLENS_CLI_CCF
+	std::cout << "--precision (0 FP64, 1 mixed), --expansion_order (0 automatic) and\n"
+				 "--m2l_mode (0-3) are this engine's additions.\n";
+#endif
 }
 
 bool present(int argc, char** argv, const char* a, const char* b)

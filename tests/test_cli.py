@@ -1,5 +1,6 @@
-"""The command-line programs irs_amd/bin/ipm, ipm_double and irs: the
-reference's options, messages and exit codes (src/main_ipm_irs.cu:41-606)."""
+"""The command-line programs irs_amd/bin/ipm, ipm_double, irs and ccf: the
+reference's options, messages and exit codes (src/main_ipm_irs.cu:41-606,
+src/main_ccf.cu:29-523)."""
 
 import os
 import subprocess
@@ -16,7 +17,7 @@ def run(prog, *args, cwd=None):
     return r.returncode, r.stdout, r.stderr
 
 
-@pytest.mark.parametrize("prog", ["ipm", "ipm_double", "irs"])
+@pytest.mark.parametrize("prog", ["ipm", "ipm_double", "irs", "ccf"])
 def test_help_and_syntax(prog):
     rc, out, _ = run(prog, "-h")
     assert rc == 255 and "Usage:" in out and "--kappa_tot" in out and "--outfile_prefix" in out
@@ -52,6 +53,40 @@ def test_option_semantics(gpu_available):
     assert "kappa_star set to: 0.1" in out and "smooth_fraction set to" not in out
     _, out, _ = run("ipm", "-v", "2", "-sf", "stars.bin", "-ks", "0.1", "-mf", "salpeter", "-k", "0.4")
     assert "kappa_star set to" not in out and "mass_function set to" not in out and "kappa_tot set to: 0.4" in out
+
+
+def test_ccf_options():
+    # ccf.cuh:42-62 defaults; the CCF options of main_ccf.cu:29-55; validation before the device
+    _, out, _ = run("ccf", "-h")
+    for text in ("num_stars. Default value: 137", "num_phi. Default value: 100", "num_branches. Default value: 1",
+                 "--write_critical_curves", "--write_length_scales"):
+        assert text in out
+    rc, _, err = run("ccf", "-np", "7", "-v", "0")
+    assert rc == 255 and "num_phi must be an even integer > 0" in err
+    rc, _, err = run("ccf", "-np", "10", "-nb", "3", "-v", "0")
+    assert rc == 255 and "num_phi must be a multiple of 2 * num_branches" in err
+    rc, out, _ = run("ccf", "-v", "2", "-sf", "stars.bin", "-ns", "5", "-ks", "0.1", "-np", "7")
+    assert "num_stars set to" not in out and "kappa_star set to" not in out and "num_phi set to: 7" in out
+
+
+@pytest.mark.gpu
+def test_cli_ccf_matches_library(tmp_path):
+    """ccf writes the critical curves and caustics the library computes for the same parameters."""
+    from irs_amd.microlensing.CCF.ccf import CCF
+    args = ["-k", "0.4", "-y", "0.3", "-ks", "0.35", "-ns", "12", "-np", "10", "-rs", "9", "-wls", "1",
+            "-o", str(tmp_path) + "/", "-v", "0"]
+    print("[gpu] cli ccf", flush=True)
+    rc, out, err = run("ccf", *args)
+    assert rc == 0, err
+    c = CCF(kappa_tot=0.4, shear=0.3, kappa_star=0.35, num_stars=12, num_phi=10, random_seed=9,
+            write_length_scales=True, write_stars=False)
+    c.run()
+    for name, arr in (("ccf_ccs.bin", c.critical_curves), ("ccf_caustics.bin", c.caustics)):
+        vals = np.fromfile(str(tmp_path / name), dtype=np.float64, offset=8).reshape(arr.shape)
+        assert np.array_equal(vals, arr)
+    mu = np.fromfile(str(tmp_path / "ccf_mu_length_scales.bin"), dtype=np.float64, offset=8)
+    assert np.array_equal(mu, c.mu_length_scales.ravel())
+    c.close()
 
 
 @pytest.mark.gpu
