@@ -15,6 +15,7 @@
 #include <cstring>
 #include <chrono>
 #include <iostream>
+#include <memory>
 #include <type_traits>
 #include <vector>
 
@@ -721,8 +722,11 @@ its arguments, including its own tables and map, are read from device memory.
 --------------------------------------------------------------------------- */
 struct BatchMap
 {
-	float* map;
-	long long npix;
+	float* map;         // row-major n2 x n1, written by the fold
+	float* tiled;       // tiled accumulation map the shooting kernel writes (MapTile<float>)
+	long long npix;     // n1 * n2
+	long long ntiled;   // elements of the tiled map
+	int n1, tiles_x;
 };
 
 __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_batch(const ShootArgs* __restrict__ args,
@@ -752,11 +756,11 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_batch(const ShootArgs* _
 		const long long ntx = (a.N1 + IPM_TILE_W - 1) / IPM_TILE_W;
 		const long long ty = local / ntx, tx = local - ty * ntx;
 		const long long J0 = ty * tile_h, J1 = min(J0 + (long long)tile_h, a.N2);
-		ipm_tile<PREC_MIXED, false, false>(a, queues[threadIdx.x >> 6], tx, J0, J1);
+		ipm_tile<PREC_MIXED, false, true>(a, queues[threadIdx.x >> 6], tx, J0, J1);
 	}
 }
 
-// zero every instance's map (blockIdx.y = instance), its histogram and the work counter
+// zero every instance's tiled map (blockIdx.y = instance), its histogram and the work counter
 __global__ void k_batch_zero(const BatchMap* __restrict__ maps, unsigned* hist, unsigned long long* counter)
 {
 	const int inst = blockIdx.y;
@@ -766,8 +770,21 @@ __global__ void k_batch_zero(const BatchMap* __restrict__ maps, unsigned* hist, 
 		if (inst == 0 && threadIdx.x == 0) *counter = 0;
 	}
 	const BatchMap m = maps[inst];
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m.ntiled; i += (long long)gridDim.x * blockDim.x)
+		m.tiled[i] = 0.0f;
+}
+
+// every instance's row-major map from its tiled map (blockIdx.y = instance); the tiled map is cleared
+__global__ void k_batch_fold(const BatchMap* __restrict__ maps)
+{
+	const BatchMap m = maps[blockIdx.y];
 	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m.npix; i += (long long)gridDim.x * blockDim.x)
-		m.map[i] = 0.0f;
+	{
+		const int y = int(i / m.n1), x = int(i - (long long)y * m.n1);
+		const long long t = tiled_index<float>(m.tiles_x, x, y);
+		m.map[i] = m.tiled[t];
+		m.tiled[t] = 0.0f;
+	}
 }
 
 // 128-bin histogram of log10(magnification) over [-2, 2] per instance; values
@@ -1666,6 +1683,7 @@ struct BatchShooter::Impl
 	std::vector<ShootArgs> args;
 	std::vector<BatchMap> maps;
 	std::vector<long long> tiles_x;   // per instance: column tiles
+	std::vector<std::unique_ptr<DeviceBuffer<char>>> tiled;   // per instance: tiled accumulation map
 	bool dirty = true;
 	int tile_h = IPM_TILE_H, bpc = 3, ncu = 0;
 	double tune_ms = 0;
@@ -1728,6 +1746,14 @@ struct BatchShooter::Impl
 		MLENS_HIP_CHECK(hipGetLastError());
 	}
 
+	void launch_fold()
+	{
+		const unsigned bx = unsigned(std::min<long long>(1024, std::max<long long>(1, (max_npix + 255) / 256)));
+		dim3 grid(bx, unsigned(args.size()));
+		k_batch_fold<<<grid, 256, 0, stream>>>(reinterpret_cast<const BatchMap*>(d_maps.ptr));
+		MLENS_HIP_CHECK(hipGetLastError());
+	}
+
 	void launch_hist()
 	{
 		const unsigned bx = unsigned(std::min<long long>(256, std::max<long long>(1, (max_npix + 1023) / 1024)));
@@ -1786,6 +1812,7 @@ struct BatchShooter::Impl
 		MLENS_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
 		launch_zero();
 		launch_shoot(tile_h, bpc, 1);
+		launch_fold();
 		launch_hist();
 		MLENS_HIP_CHECK(hipStreamEndCapture(stream, &graph));
 		MLENS_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1820,9 +1847,21 @@ int BatchShooter::add(Engine* e, const LensParams& lens, const GridParams& g)
 	ShootArgs a;
 	e->fill_batch_args(&a, lens, g);
 	if (a.N1 <= 0 || a.N2 <= 0) throw HipError("a batched instance has an empty lattice", __FILE__, __LINE__);
+	// the kernel accumulates into a tiled map of its own; the fold writes the row-major map
+	const int W = MapTile<float>::W, H = MapTile<float>::H;
+	const int tx = (a.n1 + W - 1) / W;
+	const size_t tbytes = size_t(tx) * size_t((a.n2 + H - 1) / H) * 128;
+	auto buf = std::make_unique<DeviceBuffer<char>>();
+	buf->alloc(tbytes);
+	MLENS_HIP_CHECK(hipMemset(buf->ptr, 0, tbytes));
+	BatchMap bm{reinterpret_cast<float*>(a.map), reinterpret_cast<float*>(buf->ptr), (long long)a.n1 * a.n2,
+		(long long)(tbytes / sizeof(float)), a.n1, tx};
+	a.map = buf->ptr;
+	a.tiles_x = tx;
+	impl_->tiled.push_back(std::move(buf));
 	impl_->engines.push_back(e);
 	impl_->args.push_back(a);
-	impl_->maps.push_back(BatchMap{reinterpret_cast<float*>(a.map), (long long)a.n1 * a.n2});
+	impl_->maps.push_back(bm);
 	impl_->tiles_x.push_back((a.N1 + IPM_TILE_W - 1) / IPM_TILE_W);
 	impl_->max_npix = std::max(impl_->max_npix, (long long)a.n1 * a.n2);
 	impl_->dirty = true;
