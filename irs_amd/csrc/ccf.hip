@@ -8,9 +8,10 @@
 // from positions next to the stars, then stepping phi outwards in both
 // directions with the previous phase's roots as starting points.
 //
-// On this GPU one block of 256 lanes updates 256 roots of one phase. The sums
-// over all roots and all stars of every update stream through LDS tiles, so a
-// root update costs O(roots + stars) arithmetic with one global read per tile
+// On this GPU eight lanes update one root, a block of 256 lanes 32 roots of one
+// phase. The sums over all roots and all stars of every update stream through
+// LDS tiles, split over the root's eight lanes, so a root update costs
+// O((roots + stars) / 8) arithmetic per lane with one global read per tile
 // element per block. Updates are simultaneous (Jacobi): an iteration reads one
 // copy of the roots and writes the other, so the result does not depend on
 // the order in which blocks run.
@@ -38,13 +39,19 @@ struct CCTerms
 	dcx ddzb, d2s;  // d alpha_smooth / dzbar, d^2 alpha_smooth / dzbar^2
 };
 
-// alpha_star.cuh:55-114, alpha_local.cuh:52-108, alpha_smooth.cuh:142-316
-__device__ CCTerms cc_terms(const ShootArgs& a, int node, dcx z, bool second)
+// alpha_star.cuh:55-114, alpha_local.cuh:52-108, alpha_smooth.cuh:142-316. With part < parts, only
+// the near stars whose position in the node's near list is part (mod parts) enter S1 and S2, and the
+// expansion and smooth terms are evaluated by part 0 only (zero elsewhere): the caller adds the parts.
+__device__ CCTerms cc_terms(const ShootArgs& a, int node, dcx z, bool second, int part = 0, int parts = 1)
 {
 	CCTerms t;
 	double s1x = 0, s1y = 0, s2x = 0, s2y = 0;
+	int seen = 0;
 	auto range = [&](int b, int n) {
-		for (int s = b; s < b + n; s++)
+		// the first star of this range that belongs to this part
+		const int first = ((part - seen) % parts + parts) % parts;
+		seen += n;
+		for (int s = b + first; s < b + n; s += parts)
 		{
 			const double dx = z.re - a.sx[s], dy = z.im - a.sy[s], m = a.sm[s];
 			// 1 / (z - z_i) = conj(d) / |d|^2
@@ -68,6 +75,9 @@ __device__ CCTerms cc_terms(const ShootArgs& a, int node, dcx z, bool second)
 	}
 	t.S1 = dcx{s1x, s1y};
 	t.S2 = dcx{s2x, s2y};
+	t.L1 = t.L2 = t.ddzb = t.d2s = dcx{0, 0};
+	t.ddz = 0;
+	if (part != 0) return t;
 	const double h = a.nhl[node];
 	const dcx dz = dcx{(z.re - a.ncx[node]) / h, (z.im - a.ncy[node]) / h};
 	const double* c = a.local + 2 * (long long)node * (a.p + 1);
@@ -120,10 +130,24 @@ __device__ __forceinline__ CCRow cc_row(int c, int sgn, int j, int nphi, int nb)
 
 /*
 One Aberth iteration of the roots at step j of every branch and side
-(ccf_functions.cuh:171-240, 309-386). Block (x, y): roots x*256 + [0, 256)
-of branch y / sides, side y % sides (side 0 steps phi down, side 1 up; at
-j = 0 there is one side). src and dst hold (nphi + nb) rows of nroots roots.
+(ccf_functions.cuh:171-240, 309-386). Each root is updated by CCF_PARTS
+consecutive lanes, which share its sums over the other roots, the stars (the
+poles) and its near stars, and combine them with butterfly shuffles; the
+group's first lane evaluates the expansion and smooth terms and writes the
+root. Block (x, y): roots x * CCF_ROOTS + [0, CCF_ROOTS) of branch y / sides,
+side y % sides (side 0 steps phi down, side 1 up; at j = 0 there is one
+side). src and dst hold (nphi + nb) rows of nroots roots.
 */
+constexpr int CCF_PARTS = 8;
+constexpr int CCF_ROOTS = CCF_THREADS / CCF_PARTS;
+
+__device__ __forceinline__ double group_sum(double v)
+{
+#pragma unroll
+	for (int o = CCF_PARTS / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+	return v;
+}
+
 __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const dcx* __restrict__ src, dcx* __restrict__ dst,
 	char* __restrict__ fin, int nroots, int nstars, int circle_pole, int j, int nphi, int nb, int sides)
 {
@@ -132,12 +156,13 @@ __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const 
 	const int sgn = (sides == 1) ? 0 : (b == 0 ? -1 : 1);
 	const CCRow R = cc_row(c, sgn == 0 ? 1 : sgn, sgn == 0 ? 0 : j, nphi, nb);
 	const dcx* row = src + (long long)R.row * nroots;
-	const int k = blockIdx.x * CCF_THREADS + threadIdx.x;
+	const int part = threadIdx.x % CCF_PARTS;
+	const int k = blockIdx.x * CCF_ROOTS + threadIdx.x / CCF_PARTS;
 	const bool mine = k < nroots;
 	char* done = fin + ((long long)c * 2 + b) * nroots + k;
 	const bool active = mine && !*done;
 	const dcx z = mine ? row[k] : dcx{0, 0};
-	// sums over the other roots and over the poles (stars), tile by tile
+	// this lane's share of the sums over the other roots and over the poles (stars)
 	double rx = 0, ry = 0, px = 0, py = 0;
 	for (int t0 = 0; t0 < nroots; t0 += CCF_THREADS)
 	{
@@ -148,7 +173,7 @@ __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const 
 		if (active)
 		{
 			const int n = min(CCF_THREADS, nroots - t0);
-			for (int q = 0; q < n; q++)
+			for (int q = part; q < n; q += CCF_PARTS)
 			{
 				if (t0 + q == k) continue;
 				const double dx = z.re - tx[q], dy = z.im - ty[q];
@@ -168,7 +193,7 @@ __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const 
 		if (active)
 		{
 			const int n = min(CCF_THREADS, nstars - t0);
-			for (int q = 0; q < n; q++)
+			for (int q = part; q < n; q += CCF_PARTS)
 			{
 				const double dx = z.re - tx[q], dy = z.im - ty[q];
 				const double ir2 = 1.0 / (dx * dx + dy * dy);
@@ -178,15 +203,24 @@ __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const 
 		}
 		__syncthreads();
 	}
-	if (!mine) return;
-	dcx* out = dst + (long long)R.row * nroots + k;
+	// the whole group follows the same branch: it shares its root and its done flag
 	if (!active)
 	{
-		*out = z;
+		if (mine && part == 0) dst[(long long)R.row * nroots + k] = z;
 		return;
 	}
 	const int node = find_node(a, z.re, z.im);
-	const CCTerms t = cc_terms(a, node, z, true);
+	CCTerms t = cc_terms(a, node, z, true, part, CCF_PARTS);
+	rx = group_sum(rx);
+	ry = group_sum(ry);
+	px = group_sum(px);
+	py = group_sum(py);
+	t.S1.re = group_sum(t.S1.re);
+	t.S1.im = group_sum(t.S1.im);
+	t.S2.re = group_sum(t.S2.re);
+	t.S2.im = group_sum(t.S2.im);
+	if (part != 0) return;
+	dcx* out = dst + (long long)R.row * nroots + k;
 	const dcx f0 = cc_f0(a, t, R.phi);
 	if (cc_inv_mu_error(a, t, f0) < 1e-9)
 	{
@@ -305,7 +339,7 @@ CCFResult Engine::ccf_solve(const LensParams& lens, const double* init_roots, in
 	for (int c = 0; c < nb; c++)
 		MLENS_HIP_CHECK(hipMemcpyAsync(A.ptr + (long long)(c * (2 * J + 1) + J) * nroots, init_roots + 2 * (long long)c * nroots,
 			nroots * sizeof(dcx), hipMemcpyHostToDevice, stream_));
-	const unsigned gx = unsigned((nroots + CCF_THREADS - 1) / CCF_THREADS);
+	const unsigned gx = unsigned((nroots + CCF_ROOTS - 1) / CCF_ROOTS);
 	// an even number of iterations leaves the roots in A
 	auto iterate = [&](int j, int sides, int iters) {
 		for (int it = 0; it < iters; it++)
