@@ -229,9 +229,9 @@ struct TriQueue
 	int w[IPM_QCAP];                             // box width in pixels
 	int bit[IPM_QCAP];                           // map: 0 = total or minima, 1 = saddles
 	int end[IPM_QCAP];                           // box rows, then running total
-	int ring_ey[2 * 64];                         // edge-crossed pixels awaiting their exact area: entry | map bit << 7 | row << 8
-	int ring_x[2 * 64];                          //   and their map column
-	int mark[64];                                // first slot of each entry / row starting in a pass
+	int ring_ey[4 * 64];                         // edge-crossed pixels awaiting their exact area: entry | map bit << 7 | row << 8
+	int ring_x[4 * 64];                          //   and their map column
+	int mark[2 * 64];                            // first slot of each entry / row starting in a pass
 };
 
 // a general triangle handed from the per-lane path to the wave queue
@@ -256,8 +256,8 @@ template <int PREC, bool PARITY, typename MapT, bool TILED>
 __device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>& acc, double p0x, double p0y, double p1x,
 	double p1y, double p2x, double p2y, GenTri& g)
 {
-	if (!(isfinite(p0x) && isfinite(p0y) && isfinite(p1x) && isfinite(p1y) && isfinite(p2x) && isfinite(p2y))) return false;
-	// polygon.cuh:116-144 (difference form) and ipm_functions.cuh:176
+	// polygon.cuh:116-144 (difference form) and ipm_functions.cuh:176. A corner that is not finite
+	// makes A NaN or infinite, so this test also drops every triangle with such a corner.
 	double A = ((p1y - p0y) * p0x - (p1x - p0x) * p0y + (p2y - p1y) * p1x - (p2x - p1x) * p1y + (p0y - p2y) * p2x - (p0x - p2x) * p2y) / 2;
 	if (!(fabs(A) < 10000 * a.factor) || A == 0) return false;
 	const double minx = fmin(p0x, fmin(p1x, p2x)), maxx = fmax(p0x, fmax(p1x, p2x));
@@ -307,7 +307,7 @@ __device__ __forceinline__ bool ipm_triangle(const ShootArgs& a, PixelAcc<MapT>&
 		}
 		// fractions t_b = ua / (ua - ub), t_c = ua / (ua - uc) of the edges from
 		// the lone vertex to the line; their product in one division
-		const MapT lone = MapT(double(a.factor) * ((ua * ua) / ((ua - ub) * (ua - uc))));
+		const MapT lone = MapT(double(a.factor) * (ua * ua) * dev::rcp64((ua - ub) * (ua - uc)));
 		long long klow = map_index<MapT, TILED>(a, X0, Y0);
 		long long khigh = xcut ? map_index<MapT, TILED>(a, X0 + 1, Y0) : map_index<MapT, TILED>(a, X0, Y0 + 1);
 		long long klone = lone_low ? klow : khigh, kother = lone_low ? khigh : klow;
@@ -406,7 +406,7 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	if (lane >= count) return;
 	int slot = start + lane;
 	asm volatile("" : "+v"(slot));
-	const int ey = q.ring_ey[slot & 127], X = q.ring_x[slot & 127];
+	const int ey = q.ring_ey[slot & 255], X = q.ring_x[slot & 255];
 	const int e = ey & 127, yy = ey >> 8;
 	double x[3], y[3], k[3];
 #pragma unroll
@@ -563,25 +563,33 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 		off -= cnt;
 		const int xbase = xa - off;
 		int jcarry = 0;
-		for (int k0 = 0; k0 < tot; k0 += 64)
+		// candidate pixel k of row j: the row's data, fetched from the row's lane
+		struct Cand
 		{
-			// the row of candidate pixel k0 + lane
-			q.mark[lane] = -1;
-			if (cnt > 0 && off >= k0 && off < k0 + 64) q.mark[off - k0] = lane;
-			wave_lds_sync();
-			const int j = max(wave_max_scan(q.mark[lane]), jcarry);
-			jcarry = __builtin_amdgcn_readlane(j, 63);
-			const int k = k0 + lane;
-			const int x = __shfl(xbase, j) + k;
-			const int rlim = __shfl(lim, j), rey = __shfl(ey, j);
-			const int rila = __shfl(ila, j), rira = __shfl(ira, j);
-			const long long rb = __shfl(rbase, j);
-			const MapT rfull = __shfl(full, j);
+			int k, x, lim, ey, ila, ira;
+			long long rb;
+			MapT full;
+		};
+		auto fetch = [&](int k, int j) {
+			Cand c;
+			c.k = k;
+			c.x = __shfl(xbase, j) + k;
+			c.lim = __shfl(lim, j);
+			c.ey = __shfl(ey, j);
+			c.ila = __shfl(ila, j);
+			c.ira = __shfl(ira, j);
+			c.rb = __shfl(rbase, j);
+			c.full = __shfl(full, j);
+			return c;
+		};
+		// a fully covered pixel gets the row's whole weight, a crossed one joins the ring
+		auto pixel = [&](const Cand& c) {
+			const int x = c.x, rey = c.ey;
 			bool crossed = false;
-			if (k < tot && k < rlim)
+			if (c.k < tot && c.k < c.lim)
 			{
-				if (x >= rila && x <= rira)
-					map_add<MapT, PARITY>(a, rb + map_col<MapT, TILED>(x), (rey >> 7) & 1, rfull);
+				if (x >= c.ila && x <= c.ira)
+					map_add<MapT, PARITY>(a, c.rb + map_col<MapT, TILED>(x), (rey >> 7) & 1, c.full);
 				else
 					crossed = true;
 			}
@@ -590,11 +598,28 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 			{
 				int pos = rt + int(__builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u)));
 				asm volatile("" : "+v"(pos));
-				q.ring_ey[pos & 127] = rey;
-				q.ring_x[pos & 127] = x;
+				q.ring_ey[pos & 255] = rey;
+				q.ring_x[pos & 255] = x;
 			}
 			rt += __popcll(m);
-			if (rt - rh >= 64)
+		};
+		// two passes of 64 candidates at a time: their row look-ups are independent
+		// until the carry, so their LDS, DPP and shuffle latencies overlap. The ring
+		// holds fewer than 64 slots before and at most 128 more after.
+		for (int k0 = 0; k0 < tot; k0 += 128)
+		{
+			q.mark[lane] = -1;
+			q.mark[lane + 64] = -1;
+			if (cnt > 0 && off >= k0 && off < k0 + 128) q.mark[off - k0] = lane;
+			wave_lds_sync();
+			const int ma = q.mark[lane], mb = q.mark[lane + 64];
+			const int ja = max(wave_max_scan(ma), jcarry);
+			const int jb = max(wave_max_scan(mb), __builtin_amdgcn_readlane(ja, 63));
+			jcarry = __builtin_amdgcn_readlane(jb, 63);
+			const Cand ca = fetch(k0 + lane, ja), cb = fetch(k0 + 64 + lane, jb);
+			pixel(ca);
+			if (k0 + 64 < tot) pixel(cb);
+			while (rt - rh >= 64)
 			{
 				wave_lds_sync();
 				tri_queue_exact<MapT, PARITY, TILED>(a, q, rh, 64);
