@@ -1,8 +1,9 @@
 """MI355X-native microlensing magnification-map engine (IPM and IRS).
 
 The compiled engine lives in irs_amd/bin (lib_ipm.so, lib_ipm_double.so,
-lib_irs.so, lib_surrogate.so, lib_ccf.so), built from irs_amd/csrc by `build()`. The Python interface
-mirrors the reference's `microlensing` package: see irs_amd.microlensing.
+lib_irs.so, lib_surrogate.so, lib_ccf.so, lib_ncc.so), built from irs_amd/csrc by
+`build()`. The Python interface mirrors the reference's `microlensing`
+package: see irs_amd.microlensing.
 """
 
 import os
@@ -11,7 +12,7 @@ import subprocess
 PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
 BIN_DIR = os.path.join(PACKAGE_DIR, "bin")
 CSRC_DIR = os.path.join(PACKAGE_DIR, "csrc")
-LIBRARIES = ("lib_ipm.so", "lib_ipm_double.so", "lib_irs.so", "lib_surrogate.so", "lib_ccf.so")
+LIBRARIES = ("lib_ipm.so", "lib_ipm_double.so", "lib_irs.so", "lib_surrogate.so", "lib_ccf.so", "lib_ncc.so")
 
 
 def library_path(name):

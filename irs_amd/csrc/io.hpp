@@ -211,6 +211,63 @@ inline bool write_array(const std::string& fname, const V* vals, int nrows, int 
 	return bool(out);
 }
 
+// array_functions.cuh:173-267: int32 nrows, int32 ncols, then nrows*ncols
+// complex values, double or float (told apart by the file size); returned as
+// interleaved (re, im) doubles
+inline bool read_complex_array(const std::string& fname, std::vector<double>& vals, int& nrows, int& ncols)
+{
+	if (std::filesystem::path(fname).extension() != ".bin")
+	{
+		std::cerr << "Error. File " << fname << " is not a .bin file.\n";
+		return false;
+	}
+	std::error_code err;
+	const std::uintmax_t fsize = std::filesystem::file_size(fname, err);
+	if (err)
+	{
+		std::cerr << "Error determining size of input file " << fname << "\n";
+		return false;
+	}
+	std::ifstream in(fname, std::ios_base::binary);
+	if (!in.is_open())
+	{
+		std::cerr << "Error. Failed to open file " << fname << "\n";
+		return false;
+	}
+	nrows = 0;
+	ncols = 0;
+	in.read(reinterpret_cast<char*>(&nrows), sizeof(int));
+	in.read(reinterpret_cast<char*>(&ncols), sizeof(int));
+	if (!in || nrows < 1 || ncols < 1)
+	{
+		std::cerr << "Error. File " << fname << " does not contain valid values for num_rows and num_cols.\n";
+		return false;
+	}
+	const std::uintmax_t n = std::uintmax_t(nrows) * std::uintmax_t(ncols);
+	vals.resize(size_t(2 * n));
+	if (fsize == 2 * sizeof(int) + n * 2 * sizeof(double))
+	{
+		in.read(reinterpret_cast<char*>(vals.data()), std::streamsize(2 * n * sizeof(double)));
+	}
+	else if (fsize == 2 * sizeof(int) + n * 2 * sizeof(float))
+	{
+		std::vector<float> tmp(size_t(2 * n));
+		in.read(reinterpret_cast<char*>(tmp.data()), std::streamsize(2 * n * sizeof(float)));
+		std::copy(tmp.begin(), tmp.end(), vals.begin());
+	}
+	else
+	{
+		std::cerr << "Error. Binary file does not contain valid single or double precision numbers.\n";
+		return false;
+	}
+	if (!in)
+	{
+		std::cerr << "Error. Failed to read file " << fname << "\n";
+		return false;
+	}
+	return true;
+}
+
 inline bool write_histogram(const std::string& fname, const std::vector<long long>& hist, long long minval)
 {
 	if (std::filesystem::path(fname).extension() != ".txt")

@@ -1,14 +1,17 @@
 // Command-line programs: the options, defaults, messages and exit codes of
-// the reference's executables (src/main_ipm_irs.cu:41-606, src/main_ccf.cu:29-523),
-// built as irs_amd/bin/ipm (float), ipm_double and irs (float) over the map
-// engine, and irs_amd/bin/ccf (double) over the critical-curve solver.
+// the reference's executables (src/main_ipm_irs.cu:41-606, src/main_ccf.cu:29-523,
+// src/main_ncc.cu:24-298), built as irs_amd/bin/ipm (float), ipm_double and irs
+// (float) over the map engine, irs_amd/bin/ccf (double) over the critical-curve
+// solver and irs_amd/bin/ncc (double) over the caustic-crossing counter.
 //
 // Every option takes a value: `ipm -k 0.45 -y 0.5 -s 0.2 -mf kroupa ...`.
 // Star-field options are ignored when a star file is given; -s is ignored
 // when -ks is given and otherwise sets kappa_star = (1 - s) kappa_tot after
 // all options are read.
-#ifdef MLENS_CLI_CCF
+#if defined(MLENS_CLI_CCF)
 #include "ccf_maker.hpp"
+#elif defined(MLENS_CLI_NCC)
+#include "ncc_maker.hpp"
 #else
 #include "map_maker.hpp"
 #endif
@@ -21,8 +24,13 @@
 #include <string>
 #include <vector>
 
-#if !defined(MLENS_CLI_IPM) && !defined(MLENS_CLI_IRS) && !defined(MLENS_CLI_CCF)
-#error "define MLENS_CLI_IPM, MLENS_CLI_IRS or MLENS_CLI_CCF"
+#if !defined(MLENS_CLI_IPM) && !defined(MLENS_CLI_IRS) && !defined(MLENS_CLI_CCF) && !defined(MLENS_CLI_NCC)
+#error "define MLENS_CLI_IPM, MLENS_CLI_IRS, MLENS_CLI_CCF or MLENS_CLI_NCC"
+#endif
+#if defined(MLENS_CLI_NCC)
+#define MLENS_CLI_STAR_FIELD 0
+#else
+#define MLENS_CLI_STAR_FIELD 1
 #endif
 #ifndef MLENS_CLI_DTYPE
 #define MLENS_CLI_DTYPE float
@@ -31,6 +39,8 @@
 using dtype = MLENS_CLI_DTYPE;
 #if defined(MLENS_CLI_CCF)
 using Maker = mlens::CCFMaker<dtype>;
+#elif defined(MLENS_CLI_NCC)
+using Maker = mlens::NCCMaker<dtype>;
 #elif defined(MLENS_CLI_IPM)
 using Maker = mlens::MapMaker<dtype, true>;
 #else
@@ -42,7 +52,9 @@ namespace
 
 Maker maker;
 int verbose = 1;
+#if MLENS_CLI_STAR_FIELD
 dtype smooth_fraction = static_cast<dtype>(1 - maker.kappa_star / maker.kappa_tot);
+#endif
 
 template <typename V>
 std::string show(const V& v)
@@ -89,7 +101,37 @@ Option opt(const char* s, const char* l, const char* name, Kind k, bool star, F&
 	return Option{s, l, name, k, star, std::forward<F>(set), std::move(get)};
 }
 
-#ifdef MLENS_CLI_CCF
+#if defined(MLENS_CLI_NCC)
+// main_ncc.cu:24-39
+std::vector<Option> options()
+{
+	Maker& m = maker;
+	return {
+		opt("-ip", "--infile_prefix", "infile_prefix", TEXT, false,
+			[&](double, int, const std::string& s) { m.infile_prefix = s; }, [&] { return m.infile_prefix; }),
+		opt("-cy1", "--center_y1", "center_y1", REAL, false, [&](double d, int, const std::string&) { m.center_y.re = dtype(d); },
+			[&] { return show(m.center_y.re); }),
+		opt("-cy2", "--center_y2", "center_y2", REAL, false, [&](double d, int, const std::string&) { m.center_y.im = dtype(d); },
+			[&] { return show(m.center_y.im); }),
+		opt("-hly1", "--half_length_y1", "half_length_y1", REAL, false,
+			[&](double d, int, const std::string&) { m.half_length_y.re = dtype(d); }, [&] { return show(m.half_length_y.re); }),
+		opt("-hly2", "--half_length_y2", "half_length_y2", REAL, false,
+			[&](double d, int, const std::string&) { m.half_length_y.im = dtype(d); }, [&] { return show(m.half_length_y.im); }),
+		opt("-npy1", "--num_pixels_y1", "num_pixels_y1", INTEGER, false,
+			[&](double, int i, const std::string&) { m.num_pixels_y.re = i; }, [&] { return show(m.num_pixels_y.re); }),
+		opt("-npy2", "--num_pixels_y2", "num_pixels_y2", INTEGER, false,
+			[&](double, int i, const std::string&) { m.num_pixels_y.im = i; }, [&] { return show(m.num_pixels_y.im); }),
+		opt("-os", "--over_sample", "over_sample", INTEGER, false, [&](double, int i, const std::string&) { m.over_sample = i; },
+			[&] { return show(m.over_sample); }),
+		opt("-wm", "--write_maps", "write_maps", INTEGER, false, [&](double, int i, const std::string&) { m.write_maps = i; },
+			[&] { return show(m.write_maps); }),
+		opt("-wh", "--write_histograms", "write_histograms", INTEGER, false,
+			[&](double, int i, const std::string&) { m.write_histograms = i; }, [&] { return show(m.write_histograms); }),
+		opt("-o", "--outfile_prefix", "outfile_prefix", TEXT, false,
+			[&](double, int, const std::string& s) { m.outfile_prefix = s; }, [&] { return m.outfile_prefix; }),
+	};
+}
+#elif defined(MLENS_CLI_CCF)
 // main_ccf.cu:29-55
 std::vector<Option> options()
 {
@@ -225,9 +267,11 @@ void usage(const char* name)
 		flags.resize(std::max<size_t>(flags.size() + 1, 25), ' ');
 		std::cout << "  " << flags << o.name << ". Default value: " << o.get() << "\n";
 	}
+#if MLENS_CLI_STAR_FIELD
 	std::cout << "Star-field options are ignored when a star file is given; -s is ignored\n"
 				 "when -ks is given.\n";
-#ifndef MLENS_CLI_CCF
+#endif
+#if !defined(MLENS_CLI_CCF) && !defined(MLENS_CLI_NCC)
 	std::cout << "--precision (0 FP64, 1 mixed), --expansion_order (0 automatic) and\n"
 				 "--m2l_mode (0-3) are this engine's additions.\n";
 #endif
@@ -311,6 +355,7 @@ int main(int argc, char* argv[])
 			return -1;
 		}
 		if (verbose >= 2) std::cout << o.name << " set to: " << o.get() << "\n";
+#if MLENS_CLI_STAR_FIELD
 		if (std::string(o.name) == "smooth_fraction")
 		{
 			if (smooth_fraction < 0)
@@ -324,12 +369,15 @@ int main(int argc, char* argv[])
 				return -1;
 			}
 		}
+#endif
 	}
+#if MLENS_CLI_STAR_FIELD
 	if (!have_starfile && !have_kappa_star && have_smooth)
 	{
 		maker.kappa_star = static_cast<dtype>((1 - smooth_fraction) * maker.kappa_tot);
 		if (verbose >= 2) std::cout << "kappa_star set to: " << maker.kappa_star << "\n";
 	}
+#endif
 	if (verbose >= 2) std::cout << "\n";
 	if (!maker.run(verbose)) return -1;
 	if (!maker.save(verbose)) return -1;

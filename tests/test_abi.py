@@ -20,13 +20,13 @@ BIN = os.path.join(ROOT, "irs_amd", "bin")
 def header_symbols(name):
     text = open(os.path.join(ROOT, "include", name)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b([a-z_][a-z0-9_]*)\s*\((?:IPMHandle|IRSHandle|IPMBatch|SurrogateHandle|CCFHandle|void|int "
+    return sorted(set(re.findall(r"\b([a-z_][a-z0-9_]*)\s*\((?:IPMHandle|IRSHandle|IPMBatch|SurrogateHandle|CCFHandle|NCCHandle|void|int "
                                  r"max_batch)", text)))
 
 
 @pytest.mark.parametrize("lib,header,least", [("lib_ipm.so", "lib_ipm.h", 60), ("lib_ipm_double.so", "lib_ipm.h", 60),
                                               ("lib_irs.so", "lib_irs.h", 60), ("lib_surrogate.so", "lib_surrogate.h", 10),
-                                              ("lib_ccf.so", "lib_ccf.h", 53)])
+                                              ("lib_ccf.so", "lib_ccf.h", 53), ("lib_ncc.so", "lib_ncc.h", 26)])
 def test_every_declared_symbol_is_exported(lib, header, least):
     syms = header_symbols(header)
     assert len(syms) >= least

@@ -1,6 +1,6 @@
-"""The command-line programs irs_amd/bin/ipm, ipm_double, irs and ccf: the
-reference's options, messages and exit codes (src/main_ipm_irs.cu:41-606,
-src/main_ccf.cu:29-523)."""
+"""The command-line programs irs_amd/bin/ipm, ipm_double, irs, ccf and ncc:
+the reference's options, messages and exit codes (src/main_ipm_irs.cu:41-606,
+src/main_ccf.cu:29-523, src/main_ncc.cu:24-298)."""
 
 import os
 import subprocess
@@ -67,6 +67,50 @@ def test_ccf_options():
     assert rc == 255 and "num_phi must be a multiple of 2 * num_branches" in err
     rc, out, _ = run("ccf", "-v", "2", "-sf", "stars.bin", "-ns", "5", "-ks", "0.1", "-np", "7")
     assert "num_stars set to" not in out and "kappa_star set to" not in out and "num_phi set to: 7" in out
+
+
+def test_ncc_options(tmp_path):
+    # ncc.cuh:28-35 defaults and the options of main_ncc.cu:24-39; errors before the device
+    rc, out, _ = run("ncc", "-h")
+    assert rc == 255
+    for text in ("--infile_prefix", "--center_y1", "half_length_y2. Default value: 5", "num_pixels_y1. Default value: 1000",
+                 "over_sample. Default value: 2", "--write_histograms", "--outfile_prefix"):
+        assert text in out
+    rc, _, err = run("ncc", "-os")
+    assert rc == 255 and "Error. Invalid input syntax." in err
+    rc, _, err = run("ncc", "-k", "0.3")
+    assert rc == 255 and "Unknown option -k" in err
+    rc, _, err = run("ncc", "-npy1", "abc")
+    assert rc == 255 and "Error. Invalid num_pixels_y1 input." in err
+    rc, _, err = run("ncc", "-os", "-1", "-v", "0")
+    assert rc == 255 and "over_sample must be an integer >= 0" in err
+    rc, out, err = run("ncc", "-v", "2", "-ip", str(tmp_path) + "/", "-cy1", "0.5")
+    assert rc == 255 and "center_y1 set to: 0.5" in out and "Unable to read caustics from file" in err
+
+
+@pytest.mark.gpu
+def test_cli_ncc_matches_library(tmp_path):
+    """ncc writes the map and histogram the library computes for the same caustics and parameters."""
+    from irs_amd.microlensing.NCC.ncc import NCC
+    from irs_amd.microlensing.Util import util
+    t = np.linspace(0, 2 * np.pi, 61)[::-1]
+    rows = np.array([1.5 * np.exp(1j * t), 0.5 + 1j + 0.8 * np.exp(1j * t)])
+    with open(tmp_path / "ccf_caustics.bin", "wb") as f:
+        np.array(rows.shape, dtype=np.int32).tofile(f)
+        np.stack([rows.real, rows.imag], axis=-1).astype(np.float64).tofile(f)
+    print("[gpu] cli ncc", flush=True)
+    out_dir = tmp_path / "cli"
+    out_dir.mkdir()
+    rc, out, err = run("ncc", "-ip", str(tmp_path) + "/", "-hly1", "3", "-hly2", "2.5", "-npy1", "90", "-npy2", "70",
+                       "-os", "1", "-o", str(out_dir) + "/", "-v", "0")
+    assert rc == 0, err
+    n = NCC(infile_prefix=str(tmp_path) + "/", half_length_y1=3, half_length_y2=2.5, num_pixels_y1=90,
+            num_pixels_y2=70, over_sample=1, write_maps=False, write_histograms=False)
+    n.run()
+    got = util.read_array(str(out_dir / "ncc_ncc.bin"), np.int32)
+    assert np.array_equal(got, n.num_caustic_crossings) and got.max() == 2
+    assert os.path.exists(out_dir / "ncc_ncc_numpixels.txt")
+    n.close()
 
 
 @pytest.mark.gpu
