@@ -267,6 +267,49 @@ def ccf_bench(num_stars=137, num_phi=100, seed=3, runs=3):
     return out
 
 
+def ncc_bench(num_phi=100, seed=3, runs=5):
+    """Number of caustic crossings at the reference's defaults (ncc.cuh:28-35:
+    1000 x 1000 pixels over +-5 Einstein radii, over_sample 2, i.e. a 4000 x
+    4000 grid of sub-pixels) on the caustics of the default critical-curve run
+    (137 stars). GPU time (HIP events) of the crossings and of the reduction,
+    and the FP64 oracle, which adds to every pixel left of every crossing as
+    the reference's kernel does, on the same caustics on the host; the maps
+    must be identical."""
+    import tempfile
+    from irs_amd.microlensing.CCF.ccf import CCF
+    from irs_amd.microlensing.NCC.ncc import NCC
+    from oracle import ncc as ON
+    tmp = tempfile.mkdtemp(prefix="ncc_bench_")
+    c = CCF(kappa_tot=0.3, shear=0.3, kappa_star=0.27, num_stars=137, num_phi=num_phi, num_branches=1,
+            random_seed=seed, write_stars=False, write_critical_curves=False, outfile_prefix=tmp + "/")
+    c.run()
+    c.save()
+    caustics = c.caustics
+    c.close()
+    t_ncc, t_red = [], []
+    for _ in range(runs + 1):
+        n = NCC(infile_prefix=tmp + "/", write_maps=False, write_histograms=False)
+        n.run()
+        t_ncc.append(n.t_ncc)
+        t_red.append(n.t_reduce)
+        gpu_map = n.num_caustic_crossings
+        n.close()
+    t_ncc, t_red = float(np.median(t_ncc[1:])), float(np.median(t_red[1:]))
+    stats = {}
+    t0 = time.perf_counter()
+    fine = ON.fine_counts(caustics, (0.0, 0.0), (5.0, 5.0), (4000, 4000), stats=stats)
+    want = ON.reduce_like_reference(fine, (1000, 1000), 2)
+    t_cpu = time.perf_counter() - t0
+    out = {"workload": "NCC 1000x1000 px, half-length 5, over_sample 2, caustics of 137 stars x 101 phases",
+           "segments": int(caustics.shape[0] * (caustics.shape[1] - 1)), "crossings": stats.get("crossings", 0),
+           "reference_pixel_atomics": stats.get("pixel_updates", 0), "t_ncc_s": t_ncc, "t_reduce_s": t_red,
+           "gpu_s": t_ncc + t_red, "max_count": int(gpu_map.max()), "identical_to_oracle": bool(np.array_equal(gpu_map, want)),
+           "cpu_oracle_s": t_cpu, "cpu_oracle_threads": 1, "cpu_kind": "port",
+           "speedup_vs_cpu": t_cpu / (t_ncc + t_red)}
+    log(f"[ncc] {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,6 +328,8 @@ def main():
                     help="instead of the map benchmark: train and evaluate the histogram surrogate")
     ap.add_argument("--ccf", action="store_true",
                     help="instead of the map benchmark: critical curves and caustics at the reference's defaults")
+    ap.add_argument("--ncc", action="store_true",
+                    help="instead of the map benchmark: number of caustic crossings at the reference's defaults")
     ap.add_argument("--m2l-stress", action="store_true",
                     help="instead of the map benchmark: MFMA M2L against FP64 M2L on the 20000^2 FMM stress workload")
     args = ap.parse_args()
@@ -298,6 +343,9 @@ def main():
         return
     if args.ccf:
         print(json.dumps({"ccf": ccf_bench()}), flush=True)
+        return
+    if args.ncc:
+        print(json.dumps({"ncc": ncc_bench()}), flush=True)
         return
     if args.batch:
         print(json.dumps({"batch": batch_bench(args.batch)}), flush=True)

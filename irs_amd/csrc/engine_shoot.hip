@@ -10,6 +10,7 @@
 // leaf's star list and polynomial are read through scalar loads and used as
 // scalar operands.
 #include "shoot_common.hpp"
+#include "wave_scan.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -419,35 +420,6 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	const double ar = dev::tri_pixel_area(x, y, k, double(X - q.X0[e]), double(yy));
 	if (ar > 0)
 		map_add<MapT, PARITY>(a, map_index<MapT, TILED>(a, X, q.Y0[e] + yy), (ey >> 7) & 1, MapT(q.scale[e] * ar));
-}
-
-/*
-Inclusive scans over the wave: within each row of 16 lanes by shifts of 1,
-2, 4 and 8, then across rows by broadcasting lane 15 into rows 1 and 3 and
-lane 31 into rows 2 and 3. A lane whose source lies outside its row, or whose
-row is masked off, takes the identity.
-*/
-__device__ __forceinline__ int wave_prefix_sum(int v)
-{
-	v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
-	v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
-	v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
-	v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
-	v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15
-	v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31
-	return v;
-}
-
-// the same for the maximum of values >= -1
-__device__ __forceinline__ int wave_max_scan(int v)
-{
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false));   // row_shr:1
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false));   // row_shr:2
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false));   // row_shr:4
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false));   // row_shr:8
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false));   // row_bcast:15
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false));   // row_bcast:31
-	return v;
 }
 
 /*

@@ -22,8 +22,8 @@ struct NCCGrid
 
 struct NCCStats
 {
-	double t_ncc_ms = 0;      // crossings and their row sums (HIP events)
-	double t_reduce_ms = 0;   // block maxima onto the final grid (HIP events)
+	double t_ncc_ms = 0;      // the crossings (HIP events)
+	double t_reduce_ms = 0;   // row sums and block maxima onto the final grid (HIP events)
 	long long crossings = 0;  // crossings recorded over the fine grid
 	long long dropped = 0;    // crossings in rows outside the grid (the reference reports each)
 	int min = 0, max = 0;     // extremes of the final map
@@ -47,7 +47,7 @@ private:
 	hipStream_t stream_ = nullptr;
 	hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev2_ = nullptr;
 	DeviceBuffer<double> caustics_;
-	DeviceBuffer<int> fine_, map_, extremes_;
+	DeviceBuffer<int> fine_, map_, extremes_, row_ext_;
 	DeviceBuffer<unsigned long long> counters_, hist_;
 	long long map_pixels_ = 0;
 };

@@ -150,10 +150,11 @@ def segment_crossings(p0, p1, hl, npix, logspace=False):
     return out
 
 
-def fine_counts(caustics, center, half_length, npix, logspace=False):
+def fine_counts(caustics, center, half_length, npix, logspace=False, stats=None):
     """Counts on the over-sampled grid npix = (columns, rows), as the
     reference's pixel-by-pixel atomics leave them. caustics: complex array
-    (rows, points) or real (rows, points, 2)."""
+    (rows, points) or real (rows, points, 2). stats, if given, receives the
+    number of crossings and of the per-pixel additions the reference makes."""
     c = np.asarray(caustics)
     if not np.iscomplexobj(c):
         c = c[..., 0] + 1j * c[..., 1]
@@ -167,6 +168,9 @@ def fine_counts(caustics, center, half_length, npix, logspace=False):
                 # a row inside the grid takes every l, a row outside none
                 if col >= 0 and 0 <= r < npix[1]:
                     num[r, :col + 1] += d
+                    if stats is not None:
+                        stats["crossings"] = stats.get("crossings", 0) + 1
+                        stats["pixel_updates"] = stats.get("pixel_updates", 0) + col + 1
     return num
 
 

@@ -224,6 +224,21 @@ def test_gpu_matches_oracle_on_edge_cases(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("npx,os_", [((8200, 4), 0), ((2, 3), 9), ((3, 2), 6), ((3, 2), 7), ((2, 2), 8)])
+def test_gpu_matches_oracle_on_wide_and_deep_grids(tmp_path, npx, os_):
+    """Final rows wider than one workgroup's LDS row (8192 pixels) and 512 x 512
+    sub-pixels per pixel take the two-pass path; 64 to 256 sub-pixels per axis
+    take the one-pass path with whole-wave maxima."""
+    center, hl = (0.1, 0.2), (5.0, 4.0)
+    rows = as_rows(curve_set(7))
+    print(f"[gpu] ncc grid {npx} over_sample={os_}", flush=True)
+    n = run_ncc(tmp_path, rows, center, hl, npx, os_)
+    want = ON.ncc_map(rows, center, hl, npx, os_)
+    assert want.max() >= 1 and np.array_equal(n.num_caustic_crossings, want)
+    n.close()
+
+
+@pytest.mark.gpu
 def test_gpu_reads_float_caustics(tmp_path):
     center, hl, npx = (-0.5, 0.25), (6.0, 5.0), (50, 70)
     rows = as_rows(curve_set(11))
