@@ -60,14 +60,19 @@ def flop_per_cell(nbar, p):
     return 10.0 * nbar + 8.0 * p + 16.0
 
 
-def cpu_baseline(m, p, seconds):
-    """FP64 oracle (CPU, OpenMP) on sampled lattice rows of the same map."""
+def oracle_mirror(m, p):
+    """The FP64 oracle's tree and FMM over the same star field, lens and lattice."""
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from common import Mirror
     t0 = time.time()
     mir = Mirror(O, m, p=p)
-    t_setup = time.time() - t0
+    return mir, time.time() - t0
+
+
+def cpu_baseline(mir, t_setup, seconds):
+    """FP64 oracle (CPU, OpenMP) on sampled lattice rows of the same map."""
+    from oracle import oracle as O
     n1, n2 = mir.num_cells
     rng = np.random.default_rng(0)
     rows_done = 0
@@ -310,6 +315,32 @@ def ncc_bench(num_phi=100, seed=3, runs=5):
     return out
 
 
+def oracle_window(m, mir, maps, x0, y0, size=512):
+    """A size x size window of the map re-shot by the FP64 oracle on the host:
+    the GPU lists every lattice cell that can reach the window (mixed-precision
+    corners, one pixel of margin), the oracle deflects those cells through its
+    own tree and FMM and rasterises them into the window, and each GPU map is
+    compared with it pixel by pixel."""
+    from oracle import oracle as O
+    t0 = time.time()
+    rec = m.window_cells(x0, y0, size, size, margin=1.0)
+    t_cells = time.time() - t0
+    masks = rec[:, 2].astype(np.uint64)
+    cells = int(sum(int(((masks >> np.uint64(k)) & np.uint64(1)).sum()) for k in range(63)))
+    t0 = time.time()
+    win = mir.ipm_window(rec, x0, y0, size, size)
+    t_oracle = time.time() - t0
+    out = {"window": [int(x0), int(y0), size, size], "records": int(len(rec)), "cells": cells,
+           "gpu_listing_s": t_cells, "oracle_s": t_oracle, "oracle_threads": O.lib().orc_num_threads(),
+           "oracle": "FP64 tree + FMM on the host, same stars and expansion order"}
+    for name, mp in maps.items():
+        e = np.abs(mp[y0:y0 + size, x0:x0 + size] - win) / np.maximum(np.abs(win), 1e-12)
+        out[f"median_rel_err_{name}"] = float(np.median(e))
+        out[f"max_rel_err_{name}"] = float(np.max(e))
+    log(f"[bench] oracle window {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -390,6 +421,9 @@ def main():
 
     out = {}
     fp64_rate = ref_rate = err_median = err_win = None
+    maps = {"mixed": mixed_map}
+    rng = np.random.default_rng(0)
+    y0, x0 = rng.integers(0, 10000 - 512, size=2)
     if not args.no_fp64:
         m.precision = 0
         m.shoot()
@@ -397,10 +431,9 @@ def main():
         log(f"[bench] fp64 same-algorithm shoot {m.t_shoot_cells:.3f} s")
         m.finish()
         fp64_map = m.magnifications
+        maps["fp64"] = fp64_map
         e = np.abs(mixed_map - fp64_map) / np.maximum(np.abs(fp64_map), 1e-300)
         err_median = float(np.median(e))
-        rng = np.random.default_rng(0)
-        y0, x0 = rng.integers(0, 10000 - 512, size=2)
         err_win = float(np.median(e[y0:y0 + 512, x0:x0 + 512]))
         out["fp64_max_rel_err"] = float(np.max(e))
         log(f"[bench] mixed vs fp64 median rel err {err_median:.3e} (512^2 window {err_win:.3e}), max {np.max(e):.3e}")
@@ -452,8 +485,13 @@ def main():
     }
     result.update(out)
     if not args.no_cpu:
+        log("[bench] CPU oracle: tree and FMM")
+        mir, t_setup = oracle_mirror(m, P_ORDER)
+        log("[bench] CPU oracle: 512^2 window")
+        result["cpu_oracle_window"] = oracle_window(m, mir, maps, x0, y0)
         log("[bench] CPU oracle baseline")
-        result["cpu_baseline"] = cpu_baseline(m, P_ORDER, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(mir, t_setup, args.cpu_seconds)
+        del mir
     else:
         result["cpu_baseline"] = None
     m.close()

@@ -145,6 +145,14 @@ long long get_cells_shot(IPMHandle* self);
 /* lens equation through the built tree at n points z (re, im pairs) into w;
    mu (may be NULL) receives the magnification, FP64 */
 bool ipm_deflect(IPMHandle* self, const double* z, int n, double* w, double* mu);
+/* for checking a window of a large map against a host FP64 re-shoot of just the
+   cells that reach it: the cells whose mixed-precision corners' pixel box meets
+   columns [x0, x0 + width) and rows [y0, y0 + height) (row 0 at the top), widened
+   by margin pixels. Writes at most capacity records of three values (lattice row,
+   first column of a 63-cell tile, bit mask of its cells) and returns the number
+   found (more than capacity: call again with more room), or -1 before build() */
+long long ipm_window_cells(IPMHandle* self, int x0, int y0, int width, int height, double margin, long long* records,
+                           long long capacity);
 /* the tree as the oracle exports it: geom[3n], info[9n], lists[40n] */
 int get_num_nodes(IPMHandle* self);
 bool ipm_export_tree(IPMHandle* self, double* geom, int* info, int* lists);

@@ -144,6 +144,11 @@ public:
 	int map_pixels(int i) const { return npix_[i]; }
 	bool maps_are_fp32() const { return map_fp32_; }
 
+	/* lattice cells whose mixed-precision corners' pixel box meets columns [wx0, wx0 + ww) and rows
+	   [wy0, wy0 + wh) widened by margin pixels: records (lattice row, first column of a 63-cell tile,
+	   mask of its cells), at most cap written to rec (3 x cap values); returns the number found */
+	long long window_cells(const LensParams& lens, const GridParams& g, int wx0, int wy0, int ww, int wh, double margin,
+		long long* rec, long long cap);
 	/* kernel arguments of this engine's IPM map (mixed precision) for a batched launch;
 	   `out` receives the engine-internal argument block, batch_args_size() bytes */
 	static size_t batch_args_size();

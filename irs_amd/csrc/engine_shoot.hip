@@ -710,6 +710,92 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long l
 	ipm_tile<PREC, PARITY, true>(a, queues[threadIdx.x >> 6], blockIdx.x, J0, J1);
 }
 
+/*
+Lattice cells that can reach a window of the map, for checking a window of a
+map too large to shoot on the host: every cell whose corners' pixel box (over
+its finite corners, so a superset of each surviving triangle's box) meets the
+window [wx0, wx1) x [wy0, wy1) in pixel coordinates (row 0 at the top). The
+walk is the mixed-precision lattice walk of k_ipm_fast. One record per lattice
+row and 63-cell tile that holds such a cell: (row, first column, mask of the
+tile's cells). count ends as the number of records; those beyond cap are not
+written.
+*/
+__global__ __launch_bounds__(64 * IPM_WAVES) void k_window_cells(ShootArgs a, long long row0, long long row1,
+	MLENS_TABLE_PARAMS, double wx0, double wx1, double wy0, double wy1, long long* __restrict__ rec, long long cap,
+	unsigned long long* __restrict__ count)
+{
+	MLENS_TABLE_REBIND(a)
+	const long long J0 = row0 + (long long)blockIdx.y * IPM_TILE_H;
+	if (J0 >= row1) return;
+	const long long J1 = min(J0 + IPM_TILE_H, row1);
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	const long long Ibase = (long long)blockIdx.x * IPM_TILE_W + wave * 63;
+	const long long I = Ibase + lane;
+	const long long Ic = I < a.N1 ? I : a.N1;
+	const double zx = a.x0 + a.stepx * double(Ic);
+	const bool cell_ok = lane < 63 && I < a.N1;
+	LeafCache cache;
+	auto corner = [&](long long J, double& px, double& py) {
+		double zy = a.y0 + a.stepy * double(J);
+		Deflection d = deflect<PREC_MIXED, false>(a, zx, zy, cache);
+		px = ((d.wx - a.cyx) + a.h1) * a.s1;
+		py = double(a.n2) - ((d.wy - a.cyy) + a.h2) * a.s2;
+	};
+	double bx, by;
+	corner(J0, bx, by);
+	double brx = __shfl_down(bx, 1), bry = __shfl_down(by, 1);
+	for (long long J = J0 + 1; J <= J1; J++)
+	{
+		double tx, ty;
+		corner(J, tx, ty);
+		const double trx = __shfl_down(tx, 1), try_ = __shfl_down(ty, 1);
+		// fmin / fmax skip NaN corners; all four NaN leaves NaN, which meets nothing
+		const double minx = fmin(fmin(tx, trx), fmin(bx, brx)), maxx = fmax(fmax(tx, trx), fmax(bx, brx));
+		const double miny = fmin(fmin(ty, try_), fmin(by, bry)), maxy = fmax(fmax(ty, try_), fmax(by, bry));
+		const bool hit = cell_ok && minx < wx1 && maxx > wx0 && miny < wy1 && maxy > wy0;
+		const unsigned long long m = __ballot(hit);
+		if (lane == 0 && m)
+		{
+			const unsigned long long slot = atomicAdd(count, 1ull);
+			if ((long long)slot < cap)
+			{
+				rec[3 * slot] = J - 1;
+				rec[3 * slot + 1] = Ibase;
+				rec[3 * slot + 2] = (long long)m;
+			}
+		}
+		bx = tx;
+		by = ty;
+		brx = trx;
+		bry = try_;
+	}
+}
+
+long long Engine::window_cells(const LensParams& lens, const GridParams& g, int wx0, int wy0, int ww, int wh,
+	double margin, long long* rec, long long cap)
+{
+	if (ww < 1 || wh < 1 || cap < 0 || !(margin >= 0))
+		throw HipError("window_cells: the window needs a positive size and margin >= 0", __FILE__, __LINE__);
+	ShootArgs a = ShootArgsBuilder::make(lens, g, false);
+	MLENS_TREE_ARGS(a)
+	DeviceBuffer<long long> drec;
+	if (cap > 0) drec.alloc(size_t(3 * cap));
+	DeviceBuffer<unsigned long long> dcount;
+	dcount.alloc(1);
+	MLENS_HIP_CHECK(hipMemsetAsync(dcount.ptr, 0, sizeof(unsigned long long), stream_));
+	dim3 grid(unsigned((a.N1 + IPM_TILE_W - 1) / IPM_TILE_W), unsigned((a.N2 + IPM_TILE_H - 1) / IPM_TILE_H));
+	k_window_cells<<<grid, 64 * IPM_WAVES, 0, stream_>>>(a, 0, a.N2, MLENS_TABLE_ARGS(a), wx0 - margin, wx0 + ww + margin,
+		wy0 - margin, wy0 + wh + margin, drec.ptr, cap, dcount.ptr);
+	MLENS_HIP_CHECK(hipGetLastError());
+	unsigned long long n = 0;
+	MLENS_HIP_CHECK(hipMemcpyAsync(&n, dcount.ptr, sizeof(n), hipMemcpyDeviceToHost, stream_));
+	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
+	const long long got = (long long)n < cap ? (long long)n : cap;
+	if (got > 0) MLENS_HIP_CHECK(hipMemcpy(rec, drec.ptr, size_t(3 * got) * sizeof(long long), hipMemcpyDeviceToHost));
+	return (long long)n;
+}
+
 /* ---------------------------------------------------------------------------
 Many maps in one launch. A persistent grid takes (instance, tile) work items
 from a global counter: item t covers tile t * stride of the concatenation of

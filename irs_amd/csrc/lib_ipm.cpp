@@ -45,6 +45,19 @@ bool ipm_deflect(IPMHandle* self, const double* z, int n, double* w, double* mu)
 		return false)
 }
 
+long long ipm_window_cells(IPMHandle* self, int x0, int y0, int width, int height, double margin, long long* records,
+	long long capacity)
+{
+	if (!self->m.engine() || !self->m.is_built())
+	{
+		std::cerr << "Error. build() must succeed before ipm_window_cells()\n";
+		return -1;
+	}
+	MLENS_GUARD(return self->m.engine()->window_cells(self->m.lens(), self->m.grid(), x0, y0, width, height, margin,
+					records, capacity),
+		return -1)
+}
+
 bool ipm_export_tree(IPMHandle* self, double* geom, int* info, int* lists)
 {
 	if (!self->m.engine()) return false;

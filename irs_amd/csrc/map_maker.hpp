@@ -232,6 +232,7 @@ public:
 	T get_mean_mass2() const { return mean_mass2_; }
 	T get_mean_mass_actual() const { return mean_mass_actual_; }
 	Engine* engine() { return engine_.get(); }
+	bool is_built() const { return built_; }
 	double mean_near_stars() const { return engine_ ? engine_->mean_near_stars(grid()) : 0; }
 	const std::vector<long long>& histogram(int which, bool log_hist) const
 	{

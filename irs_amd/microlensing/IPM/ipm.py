@@ -535,6 +535,21 @@ class IPM(object):
         wc = w[:, 0] + 1j * w[:, 1]
         return (wc, mu) if want_mu else wc
 
+    def window_cells(self, x0, y0, width, height, margin=1.0, capacity=1 << 22):
+        """The lattice cells that can reach columns [x0, x0 + width) and rows
+        [y0, y0 + height) of the map (row 0 at the top), widened by margin pixels:
+        an (n, 3) int64 array of (lattice row, first column of a 63-cell tile,
+        bit mask of the tile's cells). Needs build()."""
+        while True:
+            rec = np.zeros((capacity, 3), dtype=np.int64)
+            n = self.lib.ipm_window_cells(self.obj, int(x0), int(y0), int(width), int(height), float(margin),
+                                          rec.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), capacity)
+            if n < 0:
+                raise Exception("Error finding the cells of a window")
+            if n <= capacity:
+                return rec[:n].copy()
+            capacity = int(n)
+
     def export_tree(self):
         n = self.lib.get_num_nodes(self.obj)
         geom = np.zeros((n, 3))

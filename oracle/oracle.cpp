@@ -1452,6 +1452,62 @@ void orc_ipm_rows(void* h, const OrcLens* Lp, const OrcGrid* Gp, const long long
 	}
 }
 
+/* IPM over the cells named by window records (lattice row, first column of a
+   63-cell tile, bit mask of its cells), keeping only the area that lands in map
+   columns [wx0, wx0 + ww) and rows [wy0, wy0 + wh); window is wh x ww, its row 0
+   the map's row wy0. The triangles and their allocation are those of
+   orc_ipm_rows; the corners come from the tree (FP64). */
+void orc_ipm_window(void* h, const OrcLens* Lp, const OrcGrid* Gp, const long long* rec, long long nrec, int wx0, int wy0,
+	int ww, int wh, double* window)
+{
+	Tree* t = static_cast<Tree*>(h);
+	const OrcGrid G = *Gp;
+	Lens L{Lp->kappa, Lp->gamma, Lp->theta, Lp->kappastar, Lp->rectangular, Lp->corner_re, Lp->corner_im, Lp->approx, Lp->taylor_smooth};
+	const long long N1 = G.num_ray_threads[0];
+	const int n1 = G.num_pixels[0], n2 = G.num_pixels[1];
+	const double factor = 2 * G.ray_half_sep[0] * G.ray_half_sep[1] * n1 * n2 / (2 * G.half_length_y[0] * 2 * G.half_length_y[1]);
+#pragma omp parallel
+	{
+		Cx lo[64], hi[64];
+#pragma omp for schedule(dynamic, 64)
+		for (long long r = 0; r < nrec; r++)
+		{
+			const long long j = rec[3 * r], i0 = rec[3 * r + 1];
+			const unsigned long long mask = (unsigned long long)rec[3 * r + 2];
+			for (int b = 0; b < 64; b++)
+			{
+				const bool used = ((mask >> b) & 1ull) || (b > 0 && ((mask >> (b - 1)) & 1ull));
+				if (!used || i0 + b > N1) continue;
+				lo[b] = to_pixel(G, deflect_tree(*t, L, lattice_corner(G, i0 + b, j), false).w);
+				hi[b] = to_pixel(G, deflect_tree(*t, L, lattice_corner(G, i0 + b, j + 1), false).w);
+			}
+			for (int b = 0; b < 63; b++)
+			{
+				if (!((mask >> b) & 1ull) || i0 + b >= N1) continue;
+				Cx yy[4] = {hi[b + 1], hi[b], lo[b], lo[b + 1]};
+				bool bad[4];
+				for (int a = 0; a < 4; a++) bad[a] = std::isnan(yy[a].re) || std::isnan(yy[a].im);
+				const int tris[2][3] = {{0, 1, 2}, {2, 3, 0}};
+				for (int k = 0; k < 2; k++)
+				{
+					if (bad[tris[k][0]] || bad[tris[k][1]] || bad[tris[k][2]]) continue;
+					Poly P;
+					P.add(yy[tris[k][0]]);
+					P.add(yy[tris[k][1]]);
+					P.add(yy[tris[k][2]]);
+					double A = P.area();
+					if (!(std::fabs(A) < 10000 * factor)) continue;
+					allocate_area_among_pixels(P, factor, n1, n2, [&](size_t idx, double v) {
+						const long long y = (long long)(idx / size_t(n1)), x = (long long)(idx % size_t(n1));
+						if (x < wx0 || x >= wx0 + ww || y < wy0 || y >= wy0 + wh) return;
+						atomic_add(&window[(y - wy0) * ww + (x - wx0)], v);
+					});
+				}
+			}
+		}
+	}
+}
+
 /* IRS: shoot ray rows (irs_functions.cuh:111-212) into int counts; returns
    the number of rays received. Parity maps use the sign of mu. */
 long long orc_irs_rows(void* h, const OrcLens* Lp, const OrcGrid* Gp, const long long* rows, int nrows,

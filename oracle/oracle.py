@@ -90,6 +90,7 @@ def lib():
         L.orc_deflect_direct.argtypes = [P(d), i, P(Lens), P(d), i, P(d)]
         L.orc_triangle_alloc.argtypes = [P(d), d, i, i, P(ctypes.c_longlong), P(d), i]
         L.orc_ipm_rows.argtypes = [v, P(Lens), P(Grid), P(ctypes.c_longlong), i, i, P(d), P(d), P(d)]
+        L.orc_ipm_window.argtypes = [v, P(Lens), P(Grid), P(ctypes.c_longlong), ctypes.c_longlong, i, i, i, i, P(d)]
         L.orc_irs_rows.argtypes = [v, P(Lens), P(Grid), P(ctypes.c_longlong), i, P(i), P(i), P(i)]
         L.orc_irs_rows.restype = ctypes.c_longlong
         L.orc_irs_bin.argtypes = [d, d, d, d, i, i]
@@ -223,6 +224,15 @@ class Tree:
         if parities:
             return mn + sd, mn, sd
         return m
+
+    def ipm_window(self, lens, grid, records, x0, y0, width, height):
+        """IPM of the cells named by window records (lattice row, first tile column,
+        cell mask), kept to map columns [x0, x0 + width) and rows [y0, y0 + height)."""
+        rec = np.ascontiguousarray(records, dtype=np.int64).reshape(-1, 3)
+        win = np.zeros((height, width))
+        lib().orc_ipm_window(self.h, ctypes.byref(lens), ctypes.byref(grid), _ptr(rec, ctypes.c_longlong), len(rec),
+                             int(x0), int(y0), int(width), int(height), _ptr(win))
+        return win
 
     def irs_rows(self, lens, grid, rows, parities=False):
         n1, n2 = grid.num_pixels[0], grid.num_pixels[1]

@@ -90,6 +90,9 @@ class Mirror:
             rows = np.arange(self.grid.num_ray_threads[1])
         return self.tree.ipm_rows(self.lens, self.grid, rows, use_direct=use_direct, parities=parities)
 
+    def ipm_window(self, records, x0, y0, width, height):
+        return self.tree.ipm_window(self.lens, self.grid, records, x0, y0, width, height)
+
     def irs_map(self, rows=None, parities=False):
         if rows is None:
             rows = np.arange(self.grid.num_ray_threads[1])

@@ -171,6 +171,29 @@ def test_ipm_mixed_map_matches_oracle(oracle, c1_stars):
     m.close()
 
 
+def test_window_cells_cover_the_window(c1_fp64, c1_mirror):
+    """The cells ipm_window_cells lists are all the cells that reach the window:
+    re-shot by the oracle they give that window of the oracle's full map, and the
+    engine's map agrees with it as whole maps do."""
+    x0, y0, w, h = 30, 42, 25, 20
+    say("[gpu] window cells")
+    rec = c1_fp64.window_cells(x0, y0, w, h, margin=1.0)
+    assert len(rec) > 0 and (rec[:, 2] != 0).all()
+    full = c1_mirror.ipm_map()
+    win = c1_mirror.ipm_window(rec, x0, y0, w, h)
+    np.testing.assert_allclose(win, full[y0:y0 + h, x0:x0 + w], rtol=1e-12, atol=1e-13)
+    # the per-pixel bounds of whole maps (a window this small does not average the flux to 1e-9)
+    e = rel_err(c1_fp64.magnifications[y0:y0 + h, x0:x0 + w], win, floor=1e-12)
+    say(f"[fp64 window] rel err median {np.median(e):.2e}  max {np.max(e):.2e}")
+    assert np.median(e) < 1e-10 and np.percentile(e, 99) < 1e-6 and np.max(e) < 1e-5
+    # a too small record buffer reports the full count
+    import ctypes
+    small = np.zeros((4, 3), dtype=np.int64)
+    n = c1_fp64.lib.ipm_window_cells(c1_fp64.obj, x0, y0, w, h, 1.0,
+                                     small.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), 4)
+    assert n == len(rec)
+
+
 def test_ipm_parity_maps(oracle, c1_stars, c1_mirror):
     m = run_ipm(from_starfile(C1_SMALL, c1_stars[0]), is_double=True, expansion_order=16, write_parities=True)
     tot, mn, sd = c1_mirror.ipm_map(parities=True)

@@ -395,6 +395,30 @@ def test_ipm_conserves_image_area(oracle, c1_tree):
     assert abs(md.sum() - mt.sum()) < 1e-6 * md.sum()
 
 
+def test_ipm_window_of_all_cells_equals_the_map_window(oracle, c1_tree):
+    """orc_ipm_window given every cell (as 63-cell tiles with full masks, the
+    last tile cut at the lattice edge) reproduces that window of the row-shot map."""
+    T, stars, R = c1_tree
+    T.fmm(12)
+    d = oracle.derived_params(0.3, 0.3, 0.3, 1.0, "equal", 1.0, 1.0, 1.0, 0.001, 0, 1, 1.37, (0, 0), (10, 10),
+                              (60, 50), 2)
+    lens = lens_for(oracle, 0.3, 0.3, 0.3, R)
+    g = oracle.make_grid(d, (0.0, 0.0), (10.0, 10.0), (60, 50))
+    n1, n2 = d.num_ray_threads
+    rows = np.arange(0, n2, 7)
+    full = T.ipm_rows(lens, g, rows)
+    recs = []
+    for j in rows:
+        for i0 in range(0, n1, 63):
+            k = min(63, n1 - i0)
+            recs.append((j, i0, (1 << k) - 1 if k < 63 else (1 << 63) - 1))
+    recs = np.array([(a, b, np.int64(np.uint64(c).view(np.int64))) for a, b, c in recs], dtype=np.int64)
+    x0, y0, w, h = 17, 9, 25, 30
+    win = T.ipm_window(lens, g, recs, x0, y0, w, h)
+    assert win.sum() > 0
+    np.testing.assert_allclose(win, full[y0:y0 + h, x0:x0 + w], rtol=1e-12, atol=1e-15)
+
+
 def test_irs_binning_rules(oracle):
     # irs_functions.cuh:152-184: inclusive border, truncation, clamp, row flip
     h, n = 5.0, 10
