@@ -71,11 +71,13 @@ std::string show(const mlens::Complex<V>& v)
 	return s.str();
 }
 
+#if MLENS_CLI_STAR_FIELD
 std::string lowercase(std::string s)
 {
 	std::transform(s.begin(), s.end(), s.begin(), [](unsigned char c) { return char(std::tolower(c)); });
 	return s;
 }
+#endif
 
 enum Kind
 {
