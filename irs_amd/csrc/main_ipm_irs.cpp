@@ -333,7 +333,7 @@ int main(int argc, char* argv[])
 	}
 	const bool have_starfile = present(argc, argv, "-sf", "--starfile");
 	const bool have_kappa_star = present(argc, argv, "-ks", "--kappa_star");
-	const bool have_smooth = present(argc, argv, "-s", "--smooth_fraction");
+	[[maybe_unused]] const bool have_smooth = present(argc, argv, "-s", "--smooth_fraction");
 	for (int i = 1; i < argc; i += 2)
 	{
 		const std::string flag = argv[i];
