@@ -427,6 +427,7 @@ __global__ __launch_bounds__(256) void k_ncc_histogram(const int* __restrict__ m
 	for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
 	{
 		const int b = map[q] - vmin;
+		if (b < 0 || b >= len) continue;   // values outside [vmin, vmax] are not counted
 		if (LOCAL)
 			atomicAdd(local + b, 1u);
 		else
