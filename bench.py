@@ -35,6 +35,7 @@ C2 = dict(kappa_tot=0.45, shear=0.5, kappa_star=0.36, theta_star=1.0, mass_funct
           write_histograms=False)
 P_ORDER = 30
 PEAK_VALU_TFLOPS = 78.6     # MI355X vector FP64 = unpacked FP32 (packed FP32 instructions are not used)
+PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
 PEAK_PACKED_FP32_TFLOPS = 157.3
 
 
@@ -311,6 +312,11 @@ def ncc_bench(num_phi=100, seed=3, runs=5):
            "gpu_s": t_ncc + t_red, "max_count": int(gpu_map.max()), "identical_to_oracle": bool(np.array_equal(gpu_map, want)),
            "cpu_oracle_s": t_cpu, "cpu_oracle_threads": 1, "cpu_kind": "port",
            "speedup_vs_cpu": t_cpu / (t_ncc + t_red)}
+    # the row pass reads the 4000 x 4000 int32 sub-pixel grid once and writes the 1000 x 1000 map
+    row_pass_bytes = 4000 * 4000 * 4 + 1000 * 1000 * 4
+    achieved = row_pass_bytes / t_red / 1e9
+    out["roofline"] = {"kernel": "k_ncc_rows_to_map", "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                       "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "bytes_per_launch": row_pass_bytes}
     log(f"[ncc] {out}")
     return out
 
