@@ -171,6 +171,31 @@ def test_ipm_mixed_map_matches_oracle(oracle, c1_stars):
     m.close()
 
 
+def test_c1_full_size_maps_match_oracle(oracle, tmp_path):
+    """The small FP64 case at its full size: kappa_* = gamma = 0.3, equal masses,
+    3048 stars (seed 0), 1000 x 1000 pixels over half-length 10, 4 x 4 rays per
+    pixel (10930 x 27322 = 2.99e8 cells), p = 25. The oracle shoots the whole
+    lattice in FP64 on the host (OpenMP, about ten seconds); the FP64 map is held
+    to the whole-map bounds, the mixed map to the headline criterion."""
+    params = dict(C1_SMALL, num_pixels_y1=1000, num_pixels_y2=1000, num_rays_y=16, random_seed=0)
+    path = tmp_path / "c1_full.bin"
+    stars = write_starfile(oracle, path, params, seed=0)
+    assert len(stars) == 3048
+    say("[gpu] C1 full size, fp64")
+    a = run_ipm(from_starfile(params, path), is_double=True, expansion_order=25)
+    assert a.cells_shot == 10930 * 27322
+    say("[cpu] C1 full size, oracle map")
+    ref = Mirror(oracle, a).ipm_map()
+    assert_fp64_map(a.magnifications, ref, "C1 full fp64")
+    say("[gpu] C1 full size, mixed")
+    b = run_ipm(from_starfile(params, path), is_double=False, expansion_order=25)
+    e = rel_err(b.magnifications, ref, floor=1e-12)
+    say(f"[mixed C1 full] rel err median {np.median(e):.2e}  p99 {np.percentile(e, 99):.2e}  max {np.max(e):.2e}")
+    assert np.median(e) < 1e-4
+    a.close()
+    b.close()
+
+
 def test_window_cells_cover_the_window(c1_fp64, c1_mirror):
     """The cells ipm_window_cells lists are all the cells that reach the window:
     re-shot by the oracle they give that window of the oracle's full map, and the
