@@ -55,7 +55,7 @@ __device__ CCTerms cc_terms(const ShootArgs& a, int node, dcx z, bool second, in
 		{
 			const double dx = z.re - a.sx[s], dy = z.im - a.sy[s], m = a.sm[s];
 			// 1 / (z - z_i) = conj(d) / |d|^2
-			const double ir2 = 1.0 / (dx * dx + dy * dy);
+			const double ir2 = dev::rcp64(dx * dx + dy * dy);
 			const double ux = dx * ir2, uy = -dy * ir2;
 			const double u2x = ux * ux - uy * uy, u2y = 2 * ux * uy;
 			s1x += m * u2x;
@@ -138,7 +138,7 @@ root. Block (x, y): roots x * CCF_ROOTS + [0, CCF_ROOTS) of branch y / sides,
 side y % sides (side 0 steps phi down, side 1 up; at j = 0 there is one
 side). src and dst hold (nphi + nb) rows of nroots roots.
 */
-constexpr int CCF_PARTS = 8;
+constexpr int CCF_PARTS = 64;
 constexpr int CCF_ROOTS = CCF_THREADS / CCF_PARTS;
 
 __device__ __forceinline__ double group_sum(double v)
@@ -149,7 +149,8 @@ __device__ __forceinline__ double group_sum(double v)
 }
 
 __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const dcx* __restrict__ src, dcx* __restrict__ dst,
-	char* __restrict__ fin, int nroots, int nstars, int circle_pole, int j, int nphi, int nb, int sides)
+	char* __restrict__ fin, int* __restrict__ leaf_cache, int nroots, int nstars, int circle_pole, int j, int nphi, int nb,
+	int sides)
 {
 	__shared__ double tx[CCF_THREADS], ty[CCF_THREADS];
 	const int c = blockIdx.y / sides, b = blockIdx.y % sides;
@@ -159,7 +160,8 @@ __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const 
 	const int part = threadIdx.x % CCF_PARTS;
 	const int k = blockIdx.x * CCF_ROOTS + threadIdx.x / CCF_PARTS;
 	const bool mine = k < nroots;
-	char* done = fin + ((long long)c * 2 + b) * nroots + k;
+	const long long slot = ((long long)c * 2 + b) * nroots + k;
+	char* done = fin + slot;
 	const bool active = mine && !*done;
 	const dcx z = mine ? row[k] : dcx{0, 0};
 	// this lane's share of the sums over the other roots and over the poles (stars)
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const 
 			{
 				if (t0 + q == k) continue;
 				const double dx = z.re - tx[q], dy = z.im - ty[q];
-				const double ir2 = 1.0 / (dx * dx + dy * dy);
+				const double ir2 = dev::rcp64(dx * dx + dy * dy);
 				rx += dx * ir2;
 				ry -= dy * ir2;
 			}
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const 
 			for (int q = part; q < n; q += CCF_PARTS)
 			{
 				const double dx = z.re - tx[q], dy = z.im - ty[q];
-				const double ir2 = 1.0 / (dx * dx + dy * dy);
+				const double ir2 = dev::rcp64(dx * dx + dy * dy);
 				px += dx * ir2;
 				py -= dy * ir2;
 			}
@@ -209,7 +211,17 @@ __global__ __launch_bounds__(CCF_THREADS) void k_ccf_iterate(ShootArgs a, const 
 		if (mine && part == 0) dst[(long long)R.row * nroots + k] = z;
 		return;
 	}
-	const int node = find_node(a, z.re, z.im);
+	// the root's leaf from the previous iteration while the root stays strictly inside
+	// it (there the tree descent would end in the same leaf), else a fresh descent
+	const int cached = leaf_cache[slot];
+	bool inside = false;
+	if (cached >= 0)
+	{
+		const double h = a.nhl[cached] * (1 - 1e-12);
+		inside = fabs(z.re - a.ncx[cached]) < h && fabs(z.im - a.ncy[cached]) < h;
+	}
+	const int node = inside ? cached : find_node(a, z.re, z.im);
+	if (!inside && part == 0) leaf_cache[slot] = a.nchild0[node] < 0 ? node : -1;
 	CCTerms t = cc_terms(a, node, z, true, part, CCF_PARTS);
 	rx = group_sum(rx);
 	ry = group_sum(ry);
@@ -316,6 +328,7 @@ CCFResult Engine::ccf_solve(const LensParams& lens, const double* init_roots, in
 	// complex arrays as (re, im) pairs of doubles
 	DeviceBuffer<double> Ab, Bb, Cb, Wb, M;
 	DeviceBuffer<char> fin;
+	DeviceBuffer<int> leaf_cache;
 	DeviceBuffer<unsigned long long> err;
 	DeviceBuffer<int> nanf;
 	Ab.alloc(2 * total);
@@ -329,10 +342,12 @@ CCFResult Engine::ccf_solve(const LensParams& lens, const double* init_roots, in
 	} A{reinterpret_cast<dcx*>(Ab.ptr)}, B{reinterpret_cast<dcx*>(Bb.ptr)}, C{reinterpret_cast<dcx*>(Cb.ptr)},
 		W{reinterpret_cast<dcx*>(Wb.ptr)};
 	fin.alloc(2 * (size_t)nb * nroots);
+	leaf_cache.alloc(2 * (size_t)nb * nroots);
 	err.alloc(1);
 	nanf.alloc(1);
 	MLENS_HIP_CHECK(hipMemsetAsync(A.ptr, 0, total * sizeof(dcx), stream_));
 	MLENS_HIP_CHECK(hipMemsetAsync(fin.ptr, 0, 2 * (size_t)nb * nroots, stream_));
+	MLENS_HIP_CHECK(hipMemsetAsync(leaf_cache.ptr, 0xff, 2 * (size_t)nb * nroots * sizeof(int), stream_));   // -1: none
 	MLENS_HIP_CHECK(hipMemsetAsync(err.ptr, 0, sizeof(unsigned long long), stream_));
 	MLENS_HIP_CHECK(hipMemsetAsync(nanf.ptr, 0, sizeof(int), stream_));
 	// starting roots at the centre row of each branch
@@ -346,8 +361,8 @@ CCFResult Engine::ccf_solve(const LensParams& lens, const double* init_roots, in
 		{
 			const dcx* s = (it % 2 == 0) ? A.ptr : B.ptr;
 			dcx* d = (it % 2 == 0) ? B.ptr : A.ptr;
-			k_ccf_iterate<<<dim3(gx, unsigned(nb * sides)), CCF_THREADS, 0, stream_>>>(a, s, d, fin.ptr, nroots, nstars_,
-				circle_pole ? 1 : 0, j, nphi, nb, sides);
+			k_ccf_iterate<<<dim3(gx, unsigned(nb * sides)), CCF_THREADS, 0, stream_>>>(a, s, d, fin.ptr, leaf_cache.ptr,
+				nroots, nstars_, circle_pole ? 1 : 0, j, nphi, nb, sides);
 		}
 		if (iters % 2)
 		{
