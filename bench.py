@@ -40,6 +40,18 @@ PEAK_PACKED_FP32_TFLOPS = 157.3
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_ipm_fast_10000px_traffic.json")
+SQ_PROFILE = os.path.join(ROOT, "profiles", "k_ipm_fast_10000px_sq.json")
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4   # 256 CUs x 4 SIMDs at 2.4 GHz, 4 cycles per wave64 instruction
+
+
+def valu_instructions():
+    """VALU instructions per launch of k_ipm_fast (SQ_INSTS_VALU) from the counter
+    profile of this workload, or None."""
+    try:
+        with open(SQ_PROFILE) as f:
+            return float(json.load(f)["counters_per_launch"]["SQ_INSTS_VALU"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def measured_traffic():
@@ -479,7 +491,11 @@ def main():
                      "frac": achieved / PEAK_VALU_TFLOPS, "traffic": measured_traffic(),
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
                      "flop_per_cell": fpc, "kernel_ms": 1000.0 * float(np.mean(shoot_s)),
-                     "peak_packed_fp32": PEAK_PACKED_FP32_TFLOPS},
+                     "peak_packed_fp32": PEAK_PACKED_FP32_TFLOPS,
+                     "valu_instructions": valu_instructions(),
+                     "valu_issue_frac": (valu_instructions() / (VALU_ISSUE_PER_S * float(np.mean(shoot_s)))
+                                         if valu_instructions() else None),
+                     "valu_source": os.path.relpath(SQ_PROFILE, ROOT)},
         "shoot_rays_per_s": shoot_rate,
         "build_ms": 1000.0 * float(np.mean(build_s)),
         "fp64_rays_per_s": fp64_rate,
