@@ -294,6 +294,27 @@ def test_irs_map_matches_oracle(oracle, c1_stars):
     m.close()
 
 
+def test_irs_parity_maps_match_oracle(oracle, c1_stars):
+    """IRS with write_parities (irs_functions.cuh:186-197): each received ray
+    also counts as a minimum (mu >= 0) or a saddle (mu < 0), in FP64 against the
+    oracle's rays."""
+    m = run_irs(from_starfile(C1_SMALL, c1_stars[0]), precision=0, expansion_order=16, num_rays_y=9,
+                light_loss=0.001, write_parities=True)
+    mir = Mirror(oracle, m, stars=c1_stars[1])
+    ref, ref_min, ref_sad, got = mir.irs_map(parities=True)
+    assert m.num_rays_received == got
+    assert (m.num_rays_minima + m.num_rays_saddles == m.num_rays).all()
+    assert m.num_rays_minima.sum() > 0 and m.num_rays_saddles.sum() > 0
+    # as for the total map, a ray changes pixel only within ~1e-12 of a pixel edge; its parity
+    # changes only where |1/mu| is at the rounding level, on a critical curve
+    for what, g, r in (("total", m.num_rays, ref), ("minima", m.num_rays_minima, ref_min),
+                       ("saddles", m.num_rays_saddles, ref_sad)):
+        diff = np.abs(g.astype(np.int64) - r).sum()
+        say(f"[irs parities] {what}: {diff} rays differ")
+        assert diff <= 2, (what, diff)
+    m.close()
+
+
 def test_no_star_field_is_uniform(tmp_path):
     """One negligible star far outside the field: the lens map is linear, so
     every interior pixel has mu = 1 / ((1 - kappa)^2 - gamma^2)."""
