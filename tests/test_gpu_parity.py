@@ -196,6 +196,25 @@ def test_c1_full_size_maps_match_oracle(oracle, tmp_path):
     b.close()
 
 
+@pytest.mark.parametrize("rectangular,approx", [(0, 0), (1, 1), (1, 0)])
+def test_smooth_matter_variants_match_oracle(oracle, rectangular, approx):
+    """The exact circular sheet, and the rectangular field with the approximate
+    (Taylor) and the exact sheet (alpha_smooth.cuh:62-236), with smooth matter
+    kappa_tot - kappa_* = 0.1: deflections and FP64 maps against the oracle."""
+    params = dict(C1_SMALL, kappa_tot=0.4, kappa_star=0.3, shear=0.2, rectangular=bool(rectangular),
+                  approx=bool(approx), random_seed=3)
+    m = run_ipm(params, is_double=True, expansion_order=16)
+    mir = Mirror(oracle, m)
+    z = random_points(m, 5000, seed=9)
+    w_gpu = m.deflect(z)
+    w_tree, _ = mir.tree.deflect(mir.lens, z)
+    dw = np.max(np.abs(w_gpu - w_tree))
+    say(f"[smooth r={rectangular} a={approx}] max deflection difference {dw:.2e}")
+    assert dw < 1e-10
+    assert_fp64_map(m.magnifications, mir.ipm_map(), f"smooth r={rectangular} a={approx}")
+    m.close()
+
+
 def test_window_cells_cover_the_window(c1_fp64, c1_mirror):
     """The cells ipm_window_cells lists are all the cells that reach the window:
     re-shot by the oracle they give that window of the oracle's full map, and the
