@@ -236,6 +236,25 @@ def test_smooth_matter_variants_match_oracle(oracle, rectangular, approx):
     m.close()
 
 
+def test_off_centre_rectangular_map_matches_oracle(oracle, c1_stars):
+    """A map centred off the origin, with unequal half-lengths and pixel counts:
+    the FP64 IPM map and the IRS counts against the oracle's."""
+    params = dict(from_starfile(C1_SMALL, c1_stars[0]), center_y1=1.3, center_y2=-0.8, half_length_y1=6.0,
+                  half_length_y2=4.5, num_pixels_y1=80, num_pixels_y2=60)
+    m = run_ipm(params, is_double=True, expansion_order=16)
+    assert m.magnifications.shape == (60, 80)
+    mir = Mirror(oracle, m)
+    assert_fp64_map(m.magnifications, mir.ipm_map(), "off-centre fp64")
+    m.close()
+    r = run_irs(params, precision=0, expansion_order=16, num_rays_y=9)
+    ref, got = Mirror(oracle, r, stars=c1_stars[1]).irs_map()
+    assert r.num_rays_received == got
+    diff = np.abs(r.num_rays.astype(np.int64) - ref).sum()
+    say(f"[off-centre irs] {diff} rays differ")
+    assert diff <= 2
+    r.close()
+
+
 def test_window_cells_cover_the_window(c1_fp64, c1_mirror):
     """The cells ipm_window_cells lists are all the cells that reach the window:
     re-shot by the oracle they give that window of the oracle's full map, and the
