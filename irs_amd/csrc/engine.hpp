@@ -161,6 +161,7 @@ private:
 	void m2l_level(int lev, int p, bool f32, hipEvent_t e0, hipEvent_t e1);
 
 	hipStream_t stream_ = nullptr;
+	int num_cus_ = 0;   // compute units of the current device
 	hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 
 	int nstars_ = 0;

@@ -45,6 +45,9 @@ Engine::Engine()
 	MLENS_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 	MLENS_HIP_CHECK(hipEventCreate(&ev0_));
 	MLENS_HIP_CHECK(hipEventCreate(&ev1_));
+	int dev = 0;
+	MLENS_HIP_CHECK(hipGetDevice(&dev));
+	MLENS_HIP_CHECK(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, dev));
 }
 
 Engine::~Engine()
