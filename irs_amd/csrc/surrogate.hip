@@ -35,8 +35,6 @@ __host__ __device__ inline unsigned short to_bf16(float x)
 	return (unsigned short)(u >> 16);
 }
 
-__device__ inline float from_bf16(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
-
 /*
 C[m][n] (+)= sum_k A(m, k) B(k, n) with A(m, k) = A[m sam + k sak] and
 B(k, n) = B[k sbk + n sbn] in BF16, C row-major FP32. M, N, K are multiples of
