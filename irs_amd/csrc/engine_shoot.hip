@@ -633,33 +633,39 @@ __device__ __forceinline__ void tri_queue_push(TriQueue& q, int& qn, bool gen, c
 	qn += __popcll(m);
 }
 
-constexpr int IPM_WAVES = 4;
+// waves per block: one for the single-map kernel (2.39 s on the 10000^2 map,
+// 2.409 s with two, 2.428 s with four), four for the persistent batch kernel,
+// whose resident-block counts are tuned in blocks of four waves
+constexpr int IPM_WAVES = 1;
 constexpr int IPM_TILE_W = 63 * IPM_WAVES;
+constexpr int IPM_BATCH_WAVES = 4;
+constexpr int IPM_BATCH_TILE_W = 63 * IPM_BATCH_WAVES;
+constexpr int IPM_RESIDENT_WAVES_PER_CU = 12;   // 3 per SIMD at up to 168 VGPRs
 constexpr int IPM_TILE_H = 64;       // lattice rows a wave walks: the shortest tile
 constexpr int IPM_TILE_H_MAX = 1024;  // and the tallest
 
 /* Rows per tile for a launch of ntx tile columns over `rows` lattice rows: the
    tallest power of two in [IPM_TILE_H, IPM_TILE_H_MAX] that still gives at
-   least eight blocks per resident block slot (3 per CU). A tall tile deflects
+   least eight blocks per resident block slot. A tall tile deflects
    its first row once per 1024 rows instead of once per 64 and flushes its
    queue less often (10000^2 map: 2.475 s at 64 rows, 2.428 s at 1024); a short
    one keeps a small lattice spread over the whole device. */
 static inline int ipm_tile_rows(long long ntx, long long rows, int num_cus)
 {
-	const long long want = 8LL * 3 * (num_cus > 0 ? num_cus : 256);
+	const long long want = 8LL * (IPM_RESIDENT_WAVES_PER_CU / IPM_WAVES) * (num_cus > 0 ? num_cus : 256);
 	int h = IPM_TILE_H_MAX;
 	while (h > IPM_TILE_H && ntx * ((rows + h - 1) / h) < want) h >>= 1;
 	return h;
 }
 
-// One tile: this wave's 63 cells of lattice columns tile_x * IPM_TILE_W + wave * 63 + [0, 63),
+// One tile: this wave's 63 cells of lattice columns tile_x * 63 WAVES + wave * 63 + [0, 63),
 // cell rows [J0, J1). Every lane of the wave must call it.
-template <int PREC, bool PARITY, bool TILED>
+template <int PREC, bool PARITY, bool TILED, int WAVES>
 __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long long tile_x, long long J0, long long J1)
 {
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
-	const long long I = tile_x * IPM_TILE_W + wave * 63 + lane;
+	const long long I = tile_x * (63 * WAVES) + wave * 63 + lane;
 	const long long Ic = I < a.N1 ? I : a.N1;
 	const double zx = a.x0 + a.stepx * double(Ic);
 	const bool cell_ok = lane < 63 && I < a.N1;
@@ -722,7 +728,7 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long l
 	if (J0 >= row1) return;
 	const long long J1 = min(J0 + tile_h, row1);
 	__shared__ TriQueue queues[IPM_WAVES];
-	ipm_tile<PREC, PARITY, true>(a, queues[threadIdx.x >> 6], blockIdx.x, J0, J1);
+	ipm_tile<PREC, PARITY, true, IPM_WAVES>(a, queues[threadIdx.x >> 6], blockIdx.x, J0, J1);
 }
 
 /*
@@ -827,11 +833,11 @@ struct BatchMap
 	int n1, tiles_x;
 };
 
-__global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_batch(const ShootArgs* __restrict__ args,
+__global__ __launch_bounds__(64 * IPM_BATCH_WAVES) void k_ipm_batch(const ShootArgs* __restrict__ args,
 	const long long* __restrict__ tile_start, int ninst, int tile_h, long long total, long long stride,
 	unsigned long long* __restrict__ counter)
 {
-	__shared__ TriQueue queues[IPM_WAVES];
+	__shared__ TriQueue queues[IPM_BATCH_WAVES];
 	__shared__ long long next;
 	for (;;)
 	{
@@ -851,10 +857,10 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_batch(const ShootArgs* _
 		}
 		const ShootArgs a = args[lo];
 		const long long local = t - tile_start[lo];
-		const long long ntx = (a.N1 + IPM_TILE_W - 1) / IPM_TILE_W;
+		const long long ntx = (a.N1 + IPM_BATCH_TILE_W - 1) / IPM_BATCH_TILE_W;
 		const long long ty = local / ntx, tx = local - ty * ntx;
 		const long long J0 = ty * tile_h, J1 = min(J0 + (long long)tile_h, a.N2);
-		ipm_tile<PREC_MIXED, false, true>(a, queues[threadIdx.x >> 6], tx, J0, J1);
+		ipm_tile<PREC_MIXED, false, true, IPM_BATCH_WAVES>(a, queues[threadIdx.x >> 6], tx, J0, J1);
 	}
 }
 
@@ -1856,7 +1862,7 @@ struct BatchShooter::Impl
 	void launch_shoot(int h, int blocks_per_cu, long long stride)
 	{
 		const unsigned blocks = unsigned(ncu * blocks_per_cu);
-		k_ipm_batch<<<blocks, 64 * IPM_WAVES, 0, stream>>>(reinterpret_cast<const ShootArgs*>(d_args.ptr), d_start.ptr,
+		k_ipm_batch<<<blocks, 64 * IPM_BATCH_WAVES, 0, stream>>>(reinterpret_cast<const ShootArgs*>(d_args.ptr), d_start.ptr,
 			int(args.size()), h, total, stride, d_counter.ptr);
 		MLENS_HIP_CHECK(hipGetLastError());
 	}
@@ -1977,7 +1983,7 @@ int BatchShooter::add(Engine* e, const LensParams& lens, const GridParams& g)
 	impl_->engines.push_back(e);
 	impl_->args.push_back(a);
 	impl_->maps.push_back(bm);
-	impl_->tiles_x.push_back((a.N1 + IPM_TILE_W - 1) / IPM_TILE_W);
+	impl_->tiles_x.push_back((a.N1 + IPM_BATCH_TILE_W - 1) / IPM_BATCH_TILE_W);
 	impl_->max_npix = std::max(impl_->max_npix, (long long)a.n1 * a.n2);
 	impl_->dirty = true;
 	impl_->graph_tile_h = -1;
