@@ -5,7 +5,9 @@ published known-answer vectors, numerical quadrature, closed forms, an
 independent polygon clipper, brute-force direct sums and conservation laws.
 """
 
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -13,15 +15,19 @@ from scipy import integrate, stats
 
 from common import C1_SMALL, write_starfile
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
 
 # --------------------------------------------------------------------------- RNG
-# Random123 known-answer vectors for philox4x32-10 (kat_vectors, Salmon et al. SC'11)
-PHILOX_KAT = [
-    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
-    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
-    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
-     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
-]
+def _philox_kat():
+    """Random123 known-answer vectors for philox4x32-10 (kat_vectors, Salmon et al. SC'11)."""
+    with open(os.path.join(GOLDEN, "philox4x32_10_kat.json")) as fh:
+        vecs = json.load(fh)["vectors"]
+    word = lambda xs: tuple(int(x, 16) for x in xs)
+    return [(word(v["ctr"]), word(v["key"]), word(v["out"])) for v in vecs]
+
+
+PHILOX_KAT = _philox_kat()
 
 
 @pytest.mark.parametrize("ctr,key,expect", PHILOX_KAT)
