@@ -402,6 +402,33 @@ def test_irs_parity_maps_match_oracle(oracle, c1_stars):
     m.close()
 
 
+def test_irs_c2_full_size_mixed_vs_fp64():
+    """IRS on the 10000^2 configuration at full size (kappa 0.45, gamma 0.5,
+    smooth fraction 0.2, Kroupa 0.08-1.5, 10 rays per pixel, p = 30, light loss
+    0.01). The mixed and FP64 ray counts of the same stars differ only where a
+    ray lands within the FP32 deflection error of a pixel boundary: their L1
+    distance is below 1e-3 of all rays received. Flux conservation: the mean
+    magnification is within 3% of mu_ave = 1 / |(1 - kappa)^2 - gamma^2|."""
+    full = dict(C2_SMALL, half_length_y1=25.0, half_length_y2=25.0, num_pixels_y1=10000, num_pixels_y2=10000,
+                num_rays_y=10, random_seed=1, light_loss=0.01)
+    a = run_irs(full, precision=0, expansion_order=30)
+    fp64 = np.array(a.num_rays, dtype=np.int64)
+    mu64 = float(a.magnifications.mean())
+    shot = a.num_rays_shot
+    a.close()
+    b = run_irs(full, precision=1, expansion_order=30)
+    assert b.num_rays_shot == shot
+    mixed = np.array(b.num_rays, dtype=np.int64)
+    b.close()
+    assert mixed.shape == (10000, 10000)
+    l1 = float(np.abs(mixed - fp64).sum()) / float(fp64.sum())
+    mu_ave = 1.0 / abs((1 - 0.45) ** 2 - 0.5 ** 2)
+    say(f"[IRS C2 full] {shot} rays shot, {fp64.sum()} received; mixed vs fp64 L1 {l1:.2e}; "
+        f"mean mu {mu64:.4f} vs mu_ave {mu_ave:.4f}")
+    assert l1 < 1e-3, l1
+    assert abs(mu64 / mu_ave - 1) < 0.03, (mu64, mu_ave)
+
+
 def test_no_star_field_is_uniform(tmp_path):
     """One negligible star far outside the field: the lens map is linear, so
     every interior pixel has mu = 1 / ((1 - kappa)^2 - gamma^2)."""
