@@ -21,6 +21,14 @@ C2_SMALL = dict(kappa_tot=0.45, shear=0.5, kappa_star=0.36, theta_star=1.0, mass
                 write_histograms=False)
 
 
+# The FMM stress workload: kappa = kappa_star = 0.8, shear 0.1, no smooth
+# matter, Salpeter 0.1-1 M_sun, 20000^2 pixels over half-length 50 (3.3e6 stars).
+C3_FULL = dict(kappa_tot=0.8, shear=0.1, kappa_star=0.8, theta_star=1.0, mass_function="salpeter", m_solar=1.0,
+               m_lower=0.1, m_upper=1.0, light_loss=0.001, rectangular=False, approx=True, safety_scale=1.37,
+               center_y1=0.0, center_y2=0.0, half_length_y1=50.0, half_length_y2=50.0, num_pixels_y1=20000,
+               num_pixels_y2=20000, num_rays_y=1, random_seed=3, write_stars=False, write_maps=False,
+               write_histograms=False)
+
 def write_starfile(O, path, params, seed):
     """Oracle-generated star field (Philox, FP64) written in the reference's
     .bin layout (star.cuh:498-534), so that runs of the float and the double

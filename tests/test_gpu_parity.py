@@ -16,7 +16,7 @@ import sys
 import numpy as np
 import pytest
 
-from common import C1_SMALL, C2_SMALL, Mirror, from_starfile, random_points, rel_err, write_starfile
+from common import C1_SMALL, C2_SMALL, C3_FULL, Mirror, from_starfile, random_points, rel_err, write_starfile
 
 pytestmark = pytest.mark.gpu
 
@@ -132,6 +132,45 @@ def test_m2l_on_mfma_matches_oracle(oracle, c1_stars, c1_fp64, c1_mirror, mode, 
     w_direct = oracle.deflect_direct(c1_mirror.stars, c1_mirror.lens, z)
     assert np.max(np.abs(w - w_direct)) < m.alpha_error
     m.close()
+
+
+@pytest.mark.parametrize("p", [10, 30])
+def test_m2l_on_mfma_at_stress_size(p):
+    """The FMM stress workload at full size (3.3e6 stars, 11 levels, 7.3e7 M2L
+    pairs per build): M2L on FP64 MFMA (mode 3) reproduces the FP64
+    recurrences (mode 0) on the same tree to rounding; on FP32 MFMA (mode 2) the
+    local coefficients, relative to each node's largest, stay within single
+    precision of the products (median 1e-6, max 1e-3)."""
+    from irs_amd.microlensing.IPM.ipm import IPM
+    ref = None
+    mfma_pairs = set()
+    for mode in (0, 3, 2):
+        say(f"[gpu] stress size, p {p}, m2l_mode {mode}")
+        m = IPM(**C3_FULL, is_double=True, precision=0, expansion_order=p, m2l_mode=mode)
+        m.prepare()
+        m.build()
+        c = m.export_coeffs("local")
+        z = random_points(m, 50000, seed=13)
+        w = m.deflect(z)
+        pairs = m.m2l_pairs
+        m.close()
+        if ref is None:
+            ref = (c, w)
+            continue
+        c0, w0 = ref
+        # both MFMA modes batch the same source-target pairs (the recurrences do not count them)
+        mfma_pairs.add(pairs)
+        assert pairs > 0 and len(mfma_pairs) == 1, mfma_pairs
+        scale = np.maximum(np.max(np.abs(c0), axis=1, keepdims=True), 1e-300)
+        ec = (np.abs(c - c0) / scale).max(axis=1)
+        ew = float(np.abs(w - w0).max())
+        say(f"[gpu] p {p} mode {mode}: coefficients median {np.median(ec):.2e} max {ec.max():.2e}; "
+            f"deflection max {ew:.2e}")
+        if mode == 3:
+            assert ec.max() < 1e-11, ec.max()
+            assert ew < 1e-9, ew
+        else:
+            assert np.median(ec) < 1e-6 and ec.max() < 1e-3, (np.median(ec), ec.max())
 
 
 def test_deflection_matches_oracle(oracle, c1_fp64, c1_mirror):
