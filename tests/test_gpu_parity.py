@@ -560,3 +560,41 @@ def test_batched_maps_match_single_runs():
     batch.close()
     for m in objs:
         m.close()
+
+
+def test_batch_of_256_maps_at_full_size():
+    """The heterogeneous batch at full size: 256 instances (draw_instances, seed
+    0; sides 256 to 2048 px, 6.0e10 cells) in one graph-captured persistent
+    launch. Every histogram counts every pixel of its map once; eight instances,
+    spread over the batch, equal the same instances shot alone (FP32 map
+    atomics: summation order only)."""
+    from irs_amd.microlensing.IPM.batch import IPMBatch, draw_instances
+    from irs_amd.microlensing.IPM.ipm import IPM
+    params = draw_instances(256, seed=0)
+    batch = IPMBatch()
+    objs = []
+    for p in params:
+        m = IPM(**p)
+        m.prepare()
+        m.build()
+        batch.add(m)
+        objs.append(m)
+    say(f"[gpu] batch of {len(objs)} maps, {batch.cells:.3e} cells")
+    h = batch.run()
+    say(f"[gpu] batch: {batch.t_shoot:.3f} s, tile_h {batch.tile_h}, blocks/CU {batch.blocks_per_cu}")
+    for k, p in enumerate(params):
+        assert h[k].sum() == p["num_pixels_y1"] * p["num_pixels_y2"], k
+    for k in range(0, 256, 32):
+        p = params[k]
+        single = IPM(**p)
+        single.run()
+        ref = single.magnifications.astype(np.float64)
+        got = objs[k].magnifications.astype(np.float64)
+        e = rel_err(got, ref, floor=1e-3 * max(ref.max(), 1e-30))
+        say(f"[gpu] instance {k}: {p['num_pixels_y1']} px, max rel err vs single {np.max(e):.2e}")
+        assert np.max(e) < 2e-5, (k, np.max(e))
+        single.close()
+    batch.close()
+    for m in objs:
+        m.close()
+
