@@ -634,11 +634,11 @@ __device__ __forceinline__ void tri_queue_push(TriQueue& q, int& qn, bool gen, c
 }
 
 // waves per block: one for the single-map kernel (2.39 s on the 10000^2 map,
-// 2.409 s with two, 2.428 s with four), four for the persistent batch kernel,
-// whose resident-block counts are tuned in blocks of four waves
+// 2.409 s with two, 2.428 s with four) and for the persistent batch kernel, so
+// that no wave waits for the slowest wave of its block
 constexpr int IPM_WAVES = 1;
 constexpr int IPM_TILE_W = 63 * IPM_WAVES;
-constexpr int IPM_BATCH_WAVES = 4;
+constexpr int IPM_BATCH_WAVES = 1;
 constexpr int IPM_BATCH_TILE_W = 63 * IPM_BATCH_WAVES;
 constexpr int IPM_RESIDENT_WAVES_PER_CU = 12;   // 3 per SIMD at up to 168 VGPRs
 constexpr int IPM_TILE_H = 64;       // lattice rows a wave walks: the shortest tile
@@ -1806,7 +1806,7 @@ struct BatchShooter::Impl
 	std::vector<long long> tiles_x;   // per instance: column tiles
 	std::vector<std::unique_ptr<DeviceBuffer<char>>> tiled;   // per instance: tiled accumulation map
 	bool dirty = true;
-	int tile_h = IPM_TILE_H, bpc = 3, ncu = 0;
+	int tile_h = IPM_TILE_H, bpc = 12, ncu = 0;
 	double tune_ms = 0;
 	bool tuned = false;
 	hipStream_t stream = nullptr;
@@ -1896,7 +1896,7 @@ struct BatchShooter::Impl
 		// about an eighth of the tiles, spread over every instance
 		const long long stride = 8;
 		const int heights[3] = {32, 64, 128};
-		const int resident[3] = {2, 3, 4};
+		const int resident[3] = {8, 10, 12};   // one-wave blocks; 12 fit at up to 168 VGPRs
 		float best = -1;
 		auto t0 = std::chrono::steady_clock::now();
 		for (int h : heights)
