@@ -81,7 +81,8 @@ class IPMBatch:
         return self.lib.get_batch_blocks_per_cu(self.obj)
 
     def set_tuning(self, tile_h, blocks_per_cu):
-        """Use these tiles instead of autotuning."""
+        """Use these tiles instead of autotuning: tile_h lattice rows per tile and
+        blocks_per_cu resident blocks of one wave each per CU (12 fill a CU)."""
         if not self.lib.ipm_batch_set_tuning(self.obj, int(tile_h), int(blocks_per_cu)):
             raise ValueError("invalid batch tuning")
 
