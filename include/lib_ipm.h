@@ -113,6 +113,11 @@ int get_precision(IPMHandle* self);
    MFMA, FP32 for p <= 20 and FP64 above; 2 FP32 MFMA; 3 FP64 MFMA (default) */
 void set_m2l_mode(IPMHandle* self, int val);
 int get_m2l_mode(IPMHandle* self);
+/* lattice rows per tile of the shooting kernel: 0 (default) chooses them per
+   launch, a positive value (at most 65536) forces that height */
+void set_tile_rows(IPMHandle* self, int val);
+int get_tile_rows(IPMHandle* self);
+int get_last_tile_rows(IPMHandle* self);                  /* the height the last shoot used */
 double get_t_m2l(IPMHandle* self);          /* seconds in the M2L kernels of the last build (MFMA modes) */
 long long get_m2l_pairs(IPMHandle* self);   /* M2L (target, source) pairs of the last build (MFMA modes) */
 int get_m2l_fp32(IPMHandle* self);          /* 1 if the last build's M2L ran on FP32 MFMA */
@@ -135,6 +140,11 @@ ipm_dtype get_kappa_star_actual(IPMHandle* self);
 ipm_dtype get_mean_mass(IPMHandle* self);
 ipm_dtype get_mean_mass2(IPMHandle* self);
 ipm_dtype get_mean_mass_actual(IPMHandle* self);
+ipm_dtype get_mean_mass2_actual(IPMHandle* self);            /* star moments of the realised field (star.cuh:124-161) */
+ipm_dtype get_mean_mass2_ln_mass(IPMHandle* self);
+ipm_dtype get_mean_mass2_ln_mass_actual(IPMHandle* self);
+ipm_dtype get_m_lower_actual(IPMHandle* self);
+ipm_dtype get_m_upper_actual(IPMHandle* self);
 ipm_dtype get_mu_ave(IPMHandle* self);
 /* staged run for benchmarking: run() == prepare + build + shoot(all rows) + finish */
 bool ipm_prepare(IPMHandle* self, int verbose);       /* validate, derive, generate stars */

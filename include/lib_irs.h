@@ -104,6 +104,11 @@ int get_precision(IRSHandle* self);
    MFMA, FP32 for p <= 20 and FP64 above; 2 FP32 MFMA; 3 FP64 MFMA (default) */
 void set_m2l_mode(IRSHandle* self, int val);
 int get_m2l_mode(IRSHandle* self);
+/* lattice rows per tile of the shooting kernel: 0 (default) chooses them per
+   launch, a positive value (at most 65536) forces that height */
+void set_tile_rows(IRSHandle* self, int val);
+int get_tile_rows(IRSHandle* self);
+int get_last_tile_rows(IRSHandle* self);                  /* the height the last shoot used */
 double get_t_m2l(IRSHandle* self);          /* seconds in the M2L kernels of the last build (MFMA modes) */
 long long get_m2l_pairs(IRSHandle* self);   /* M2L (target, source) pairs of the last build (MFMA modes) */
 int get_m2l_fp32(IRSHandle* self);          /* 1 if the last build's M2L ran on FP32 MFMA */
@@ -124,10 +129,17 @@ irs_dtype get_kappa_star_actual(IRSHandle* self);
 irs_dtype get_mean_mass(IRSHandle* self);
 irs_dtype get_mean_mass2(IRSHandle* self);
 irs_dtype get_mean_mass_actual(IRSHandle* self);
+irs_dtype get_mean_mass2_actual(IRSHandle* self);            /* star moments of the realised field (star.cuh:124-161) */
+irs_dtype get_mean_mass2_ln_mass(IRSHandle* self);
+irs_dtype get_mean_mass2_ln_mass_actual(IRSHandle* self);
+irs_dtype get_m_lower_actual(IRSHandle* self);
+irs_dtype get_m_upper_actual(IRSHandle* self);
 irs_dtype get_mu_ave(IRSHandle* self);
 int get_num_nodes(IRSHandle* self);
 long long get_cells_shot(IRSHandle* self);
 bool irs_prepare(IRSHandle* self, int verbose);
+/* validation and derived geometry only (no device work), as ipm_derive */
+bool irs_derive(IRSHandle* self, int verbose);
 bool irs_build(IRSHandle* self, int verbose);
 bool irs_shoot(IRSHandle* self, int verbose, long long row0, long long row1);
 bool irs_finish(IRSHandle* self, int verbose);

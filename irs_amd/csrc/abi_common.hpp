@@ -59,6 +59,7 @@
 	MLENS_STRING_FIELD(outfile_prefix, outfile_prefix)      \
 	MLENS_FIELD(precision, int, precision)                  \
 	MLENS_FIELD(m2l_mode, int, m2l_mode)                    \
+	MLENS_FIELD(tile_rows, int, tile_rows)                  \
 	void set_expansion_order(HANDLE* self, int val) { self->m.expansion_order_override = val; } \
 	int get_expansion_order(HANDLE* self)                   \
 	{                                                       \
@@ -95,6 +96,12 @@
 	DTYPE get_mean_mass(HANDLE* self) { return self->m.get_mean_mass(); }                                    \
 	DTYPE get_mean_mass2(HANDLE* self) { return self->m.get_mean_mass2(); }                                  \
 	DTYPE get_mean_mass_actual(HANDLE* self) { return self->m.get_mean_mass_actual(); }                      \
+	DTYPE get_mean_mass2_actual(HANDLE* self) { return self->m.get_mean_mass2_actual(); }                    \
+	DTYPE get_mean_mass2_ln_mass(HANDLE* self) { return self->m.get_mean_mass2_ln_mass(); }                  \
+	DTYPE get_mean_mass2_ln_mass_actual(HANDLE* self) { return self->m.get_mean_mass2_ln_mass_actual(); }    \
+	DTYPE get_m_lower_actual(HANDLE* self) { return self->m.get_m_lower_actual(); }                          \
+	DTYPE get_m_upper_actual(HANDLE* self) { return self->m.get_m_upper_actual(); }                          \
+	int get_last_tile_rows(HANDLE* self) { return self->m.get_last_tile_rows(); }                            \
 	DTYPE get_mu_ave(HANDLE* self) { return self->m.get_mu_ave(); }                                          \
 	bool run(HANDLE* self, int verbose) { MLENS_GUARD(return self->m.run(verbose), return false) }           \
 	bool save(HANDLE* self, int verbose) { MLENS_GUARD(return self->m.save(verbose), return false) }         \

@@ -121,6 +121,11 @@ public:
 		long long row0 = 0, long long row1 = -1);
 	ShootStats shoot_irs(const LensParams& lens, const GridParams& g, Precision prec, long long row0 = 0,
 		long long row1 = -1);
+	/* lattice rows per tile of the shooting kernels: 0 picks them per launch
+	   (IPM 64-1024, IRS 32-256); a positive value forces that height */
+	void set_tile_rows(int rows) { tile_rows_ = rows; }
+	int tile_rows() const { return tile_rows_; }
+	int last_tile_rows() const { return last_tile_rows_; }   // the height of the last shooting launch
 	void add_parity_maps();
 	void download_map(int which, void* host) const;   // 0 = total, 1 = minima, 2 = saddles
 	void map_minmax(int which, double* mn, double* mx);
@@ -162,6 +167,7 @@ private:
 
 	hipStream_t stream_ = nullptr;
 	int num_cus_ = 0;   // compute units of the current device
+	int tile_rows_ = 0, last_tile_rows_ = 0;
 	hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 
 	int nstars_ = 0;

@@ -1508,7 +1508,9 @@ ShootStats Engine::shoot_ipm(const LensParams& lens, const GridParams& g, Precis
 		a.mmin = parities_ ? tmap_min_.ptr : nullptr;
 		a.msad = parities_ ? tmap_sad_.ptr : nullptr;
 		const long long ntx = (a.N1 + IPM_TILE_W - 1) / IPM_TILE_W;
-		const int th = ipm_tile_rows(ntx, row1 - row0, num_cus_);
+		const int th = tile_rows_ > 0 ? tile_rows_ : ipm_tile_rows(ntx, row1 - row0, num_cus_);
+		last_tile_rows_ = th;
+		if ((row1 - row0 + th - 1) / th > 65535) throw HipError("too many tiles of this height for one launch", __FILE__, __LINE__);
 		dim3 grid(unsigned(ntx), unsigned((row1 - row0 + th - 1) / th));
 		dim3 block(64 * IPM_WAVES);
 		if (prec == PREC_FP64)
@@ -1550,7 +1552,9 @@ ShootStats Engine::shoot_irs(const LensParams& lens, const GridParams& g, Precis
 	MLENS_HIP_CHECK(hipMemsetAsync(scratch_ll_.ptr, 0, sizeof(long long), stream_));
 	unsigned long long* rec = reinterpret_cast<unsigned long long*>(scratch_ll_.ptr);
 	const long long ntx = (a.N1 + IRS_THREADS - 1) / IRS_THREADS;
-	const int th = irs_tile_rows(ntx, row1 - row0, num_cus_);
+	const int th = tile_rows_ > 0 ? tile_rows_ : irs_tile_rows(ntx, row1 - row0, num_cus_);
+	last_tile_rows_ = th;
+	if ((row1 - row0 + th - 1) / th > 65535) throw HipError("too many tiles of this height for one launch", __FILE__, __LINE__);
 	dim3 grid(unsigned(ntx), unsigned((row1 - row0 + th - 1) / th));
 	MLENS_HIP_CHECK(hipEventRecord(ev0_, stream_));
 	if (prec == PREC_FP64)

@@ -31,6 +31,7 @@ long long get_num_rays_shot(IRSHandle* self) { return self->m.get_num_rays_shot(
 long long get_num_rays_received(IRSHandle* self) { return self->m.get_num_rays_received(); }
 
 bool irs_prepare(IRSHandle* self, int verbose) { MLENS_GUARD(return self->m.prepare(verbose), return false) }
+bool irs_derive(IRSHandle* self, int verbose) { MLENS_GUARD(return self->m.derive(verbose), return false) }
 bool irs_build(IRSHandle* self, int verbose) { MLENS_GUARD(return self->m.build(verbose), return false) }
 bool irs_shoot(IRSHandle* self, int verbose, long long row0, long long row1)
 {

@@ -64,6 +64,7 @@ public:
 	int m2l_mode = M2L_MFMA_F64;        // M2LMode: batched GEMMs on FP64 MFMA unless set otherwise
 	int kernel = KERNEL_FAST;
 	int precision = std::is_same<T, float>::value ? PREC_MIXED : PREC_FP64;
+	int tile_rows = 0;                  // lattice rows per tile of the shooting kernel; 0: chosen per launch
 
 	MapMaker() {}
 
@@ -78,6 +79,7 @@ public:
 			if (!calculate_derived_params(verbose)) return false;
 			if (!set_device(verbose)) return false;
 			if (!populate_star_array(verbose)) return false;
+			stars_stale_ = true;   // the new field, in drawing order until build() sorts it
 			prepared_ = true;
 			return true;
 		}
@@ -124,6 +126,7 @@ public:
 				return false;
 			}
 			built_ = create_tree(verbose);
+			stars_stale_ = built_;   // tree order is final once the tree exists; get_stars fetches it
 			return built_;
 		}
 		catch (const HipError& e)
@@ -145,6 +148,7 @@ public:
 			if (!check_input_params(0)) return false;
 			// (re)allocates when precision changed since prepare(); zeroes the maps
 			engine_->alloc_maps(num_pixels_y.re, num_pixels_y.im, IS_IPM, Precision(precision), write_parities != 0);
+			engine_->set_tile_rows(tile_rows);
 			LensParams L = lens();
 			GridParams G = grid();
 			MLENS_VERBOSE((IS_IPM ? "Shooting cells...\n" : "Shooting rays...\n"), verbose, 1);
@@ -203,7 +207,22 @@ public:
 	/* ---------------- getters ---------------- */
 	int get_num_stars() const { return num_stars_; }
 	Complex<T> get_corner() const { return rectangular ? corner_ : Complex<T>(corner_.abs(), 0); }   // ipm.cuh:1350
-	T* get_stars() { return stars_host_.empty() ? nullptr : stars_host_.data(); }
+	T* get_stars()
+	{
+		if (stars_stale_ && engine_)
+		{
+			try
+			{
+				download_stars();
+			}
+			catch (const HipError& e)
+			{
+				std::cerr << "Error. " << e.msg << " (" << e.file << ":" << e.line << ")\n";
+				return nullptr;
+			}
+		}
+		return stars_host_.empty() ? nullptr : stars_host_.data();
+	}
 	T* get_pixels() { return pix_.empty() ? nullptr : pix_.data(); }
 	T* get_pixels_minima() { return pix_min_.empty() ? nullptr : pix_min_.data(); }
 	T* get_pixels_saddles() { return pix_sad_.empty() ? nullptr : pix_sad_.data(); }
@@ -231,6 +250,12 @@ public:
 	T get_mean_mass() const { return mean_mass_; }
 	T get_mean_mass2() const { return mean_mass2_; }
 	T get_mean_mass_actual() const { return mean_mass_actual_; }
+	T get_mean_mass2_actual() const { return mean_mass2_actual_; }
+	T get_mean_mass2_ln_mass() const { return mean_mass2_ln_mass_; }
+	T get_mean_mass2_ln_mass_actual() const { return mean_mass2_ln_mass_actual_; }
+	T get_m_lower_actual() const { return m_lower_actual_; }
+	T get_m_upper_actual() const { return m_upper_actual_; }
+	int get_last_tile_rows() const { return engine_ ? engine_->last_tile_rows() : 0; }
 	Engine* engine() { return engine_.get(); }
 	bool is_built() const { return built_; }
 	double mean_near_stars() const { return engine_ ? engine_->mean_near_stars(grid()) : 0; }
@@ -301,6 +326,7 @@ private:
 	StarFileData file_stars_;
 
 	std::vector<T> stars_host_;
+	bool stars_stale_ = false;
 	std::vector<T> pix_, pix_min_, pix_sad_;
 	std::vector<int> ipix_, ipix_min_, ipix_sad_;
 	long long min_mag_ = 0, max_mag_ = 0, min_log_mag_ = 0, max_log_mag_ = 0;
@@ -419,6 +445,11 @@ private:
 		if (m2l_mode < M2L_DIRECT || m2l_mode > M2L_MFMA_F64)
 		{
 			std::cerr << "Error. m2l_mode must be 0 (direct FP64), 1 (MFMA, FP32 for p <= 20), 2 (FP32 MFMA) or 3 (FP64 MFMA).\n";
+			return false;
+		}
+		if (tile_rows < 0 || tile_rows > 65536)
+		{
+			std::cerr << "Error. tile_rows must be 0 (chosen per launch) or between 1 and 65536.\n";
 			return false;
 		}
 		if (kernel != KERNEL_FAST && kernel != KERNEL_REFERENCE)
@@ -705,8 +736,11 @@ private:
 		{
 			double mn, mx;
 			engine_->map_minmax(k, &mn, &mx);
-			long long a = IS_IPM ? std::llround(T(mn) * factor) : (long long)mn;
-			long long b = IS_IPM ? std::llround(T(mx) * factor) : (long long)mx;
+			// in double, as k_histogram bins every pixel (the reference rounds these two in T
+			// and its pixels in double, ipm.cuh:972-973 and array_functions.cuh:127, so a float
+			// map's extreme pixel can fall outside its own histogram there)
+			long long a = IS_IPM ? std::llround(double(T(mn)) * factor) : (long long)mn;
+			long long b = IS_IPM ? std::llround(double(T(mx)) * factor) : (long long)mx;
 			T mu_min_actual = IS_IPM ? T(1.0 * a / factor) : T(1.0 * a / num_rays_y);
 			if (k < 2 && mu_ave_ > 1 && mu_min_actual < mu_min_theory)
 			{
@@ -736,8 +770,8 @@ private:
 				engine_->map_minmax(k, &mn, &mx);
 				double pos_min = mn > 0 ? mn : positive_min(k);
 				if (!(pos_min > 0) || !(mx > 0)) continue;
-				llo = std::min(llo, (long long)std::llround(std::log10(T(pos_min)) * factor));
-				lhi = std::max(lhi, (long long)std::llround(std::log10(T(mx)) * factor));
+				llo = std::min(llo, (long long)std::llround(std::log10(double(T(pos_min))) * factor));
+				lhi = std::max(lhi, (long long)std::llround(std::log10(double(T(mx))) * factor));
 			}
 			if (lhi >= llo)
 			{
@@ -772,6 +806,15 @@ private:
 		for (double x : v)
 			if (x > 0 && x < m) m = x;
 		return m;
+	}
+
+	// the star field in tree order, as get_stars returns it
+	void download_stars()
+	{
+		std::vector<double> s(size_t(num_stars_) * 3);
+		engine_->download_stars(s.data());
+		stars_host_.assign(s.begin(), s.end());
+		stars_stale_ = false;
 	}
 
 	void download_results()
@@ -833,9 +876,7 @@ private:
 				ipix_sad_.clear();
 			}
 		}
-		std::vector<double> s(size_t(num_stars_) * 3);
-		engine_->download_stars(s.data());
-		stars_host_.assign(s.begin(), s.end());
+		download_stars();
 	}
 
 	/* ipm.cuh:1122-1322 */

@@ -6,7 +6,8 @@ lib_ipm.so / lib_ipm_double.so.
 Additions: `expansion_order` (force the FMM order), `precision` (0 = FP64,
 1 = mixed), `kernel` (0 = fast, 1 = reference-faithful FP64), `m2l_mode`
 (0 = FP64 recurrences, 1 = MFMA with FP32 up to p = 20 and FP64 above,
-2 = FP32 MFMA, 3 = FP64 MFMA) with `t_m2l`/`m2l_pairs`, the staged
+2 = FP32 MFMA, 3 = FP64 MFMA) with `t_m2l`/`m2l_pairs`, `tile_rows` (lattice
+rows per tile of the shooting kernel; 0 = chosen per launch), the staged
 `prepare/build/shoot/finish` used by the benchmark, `deflect` for point
 evaluations, and `close()`, which releases the engine object (the reference
 never frees it).
@@ -30,7 +31,8 @@ class IPM(object):
                  num_rays_y: int = None, random_seed: int = None, write_stars: bool = None,
                  write_maps: bool = None, write_parities: bool = None, write_histograms: bool = None,
                  outfile_prefix: str = None, verbose: int = 0, is_double: bool = False,
-                 expansion_order: int = None, precision: int = None, kernel: int = None, m2l_mode: int = None):
+                 expansion_order: int = None, precision: int = None, kernel: int = None, m2l_mode: int = None,
+                 tile_rows: int = None):
         """Arguments as in the reference (ipm.py:15-45); see the class docstring for the additions."""
         self.is_double = is_double
         self.lib = self._library(is_double)
@@ -74,6 +76,7 @@ class IPM(object):
         self.precision = precision
         self.kernel = kernel
         self.m2l_mode = m2l_mode
+        self.tile_rows = tile_rows
 
         self.magnifications = None
         self.magnifications_minima = None
@@ -421,6 +424,22 @@ class IPM(object):
             self.lib.set_m2l_mode(self.obj, int(value))
 
     @property
+    def tile_rows(self):
+        return self.lib.get_tile_rows(self.obj)
+
+    @tile_rows.setter
+    def tile_rows(self, value):
+        if value is not None:
+            if value < 0 or value > 65536:
+                raise ValueError("tile_rows must be 0 (chosen per launch) or between 1 and 65536")
+            self.lib.set_tile_rows(self.obj, int(value))
+
+    @property
+    def last_tile_rows(self):
+        """Lattice rows per tile of the last shooting launch."""
+        return self.lib.get_last_tile_rows(self.obj)
+
+    @property
     def t_m2l(self):
         """Seconds in the MFMA M2L kernels of the last build."""
         return self.lib.get_t_m2l(self.obj)
@@ -511,6 +530,7 @@ class IPM(object):
     def build(self):
         if not self.lib.ipm_build(self.obj, self.verbose):
             raise Exception("Error building the IPM tree")
+        self.stars = np.ctypeslib.as_array(self.lib.get_stars(self.obj), shape=(self.num_stars, 3)).copy()
 
     def shoot(self, row0=0, row1=-1):
         if not self.lib.ipm_shoot(self.obj, self.verbose, int(row0), int(row1)):

@@ -72,6 +72,10 @@ class IRS(IPM):
             raise Exception("Error running IRS")
         self._copy_results()
 
+    def derive(self):
+        if not self.lib.irs_derive(self.obj, self.verbose):
+            raise Exception("Error deriving IRS parameters")
+
     def prepare(self):
         if not self.lib.irs_prepare(self.obj, self.verbose):
             raise Exception("Error preparing IRS")
@@ -79,6 +83,7 @@ class IRS(IPM):
     def build(self):
         if not self.lib.irs_build(self.obj, self.verbose):
             raise Exception("Error building the IRS tree")
+        self.stars = np.ctypeslib.as_array(self.lib.get_stars(self.obj), shape=(self.num_stars, 3)).copy()
 
     def shoot(self, row0=0, row1=-1):
         if not self.lib.irs_shoot(self.obj, self.verbose, int(row0), int(row1)):

@@ -16,13 +16,14 @@ _FIELDS = [
     ("half_length_y1", "d"), ("half_length_y2", "d"), ("num_pixels_y1", "i"), ("num_pixels_y2", "i"),
     ("num_rays_y", "i"), ("random_seed", "i"), ("write_stars", "i"), ("write_maps", "i"),
     ("write_parities", "i"), ("write_histograms", "i"), ("outfile_prefix", "s"),
-    ("expansion_order", "i"), ("precision", "i"), ("m2l_mode", "i"),
+    ("expansion_order", "i"), ("precision", "i"), ("m2l_mode", "i"), ("tile_rows", "i"),
 ]
 
 
 _DERIVED = ["get_center_x1", "get_center_x2", "get_half_length_x1", "get_half_length_x2", "get_ray_half_sep_1",
             "get_ray_half_sep_2", "get_kappa_star_actual", "get_mean_mass", "get_mean_mass2", "get_mean_mass_actual",
-            "get_mu_ave"]
+            "get_mu_ave", "get_mean_mass2_actual", "get_mean_mass2_ln_mass", "get_mean_mass2_ln_mass_actual",
+            "get_m_lower_actual", "get_m_upper_actual"]
 
 
 def load(path, dtype, irs=False):
@@ -62,6 +63,7 @@ def load(path, dtype, irs=False):
     sig("get_t_m2l", [v], ctypes.c_double)
     sig("get_m2l_pairs", [v], ll)
     sig("get_m2l_fp32", [v], i)
+    sig("get_last_tile_rows", [v], i)
     for name in _DERIVED:
         sig(name, [v], d)
     sig("run", [v, i], b)
@@ -75,6 +77,7 @@ def load(path, dtype, irs=False):
         sig("get_num_rays_shot", [v], ll)
         sig("get_num_rays_received", [v], ll)
         sig("irs_prepare", [v, i], b)
+        sig("irs_derive", [v, i], b)
         sig("irs_build", [v, i], b)
         sig("irs_shoot", [v, i, ll, ll], b)
         sig("irs_finish", [v, i], b)
@@ -111,7 +114,8 @@ def load(path, dtype, irs=False):
     return lib
 
 
-_M2L_SYMBOLS = ["set_m2l_mode", "get_m2l_mode", "get_t_m2l", "get_m2l_pairs", "get_m2l_fp32"]
+_M2L_SYMBOLS = ["set_m2l_mode", "get_m2l_mode", "get_t_m2l", "get_m2l_pairs", "get_m2l_fp32", "set_tile_rows",
+                "get_tile_rows", "get_last_tile_rows"]
 
 # every symbol include/lib_ipm.h declares (the first 63 are the reference's)
 IPM_SYMBOLS = (["init"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f"get_{n}" for n, _ in _FIELDS[:26]] +
@@ -132,5 +136,5 @@ IRS_SYMBOLS = (["init", "irs_free"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f
                 "set_expansion_order", "get_expansion_order", "set_precision", "get_precision", "get_tree_levels",
                 "get_alpha_error", "get_num_ray_threads_1", "get_num_ray_threads_2", "get_root_half_length",
                 "get_mean_near_stars", "get_num_nodes", "get_cells_shot", "get_taylor_smooth"] + _DERIVED +
-               ["irs_prepare", "irs_build", "irs_shoot",
+               ["irs_prepare", "irs_derive", "irs_build", "irs_shoot",
                 "irs_finish", "irs_ray_positions"] + _M2L_SYMBOLS)

@@ -1107,6 +1107,11 @@ struct OrcDerived
 	double corner[2];
 	double alpha_error;
 	int taylor_smooth;
+	// the lens the geometry was derived for: the inputs, or the star file's
+	// values (ipm.cuh:405-432) when the stars come from a file
+	double kappa_star, theta_star;
+	int rectangular;
+	int covered;   // 0 when a star file's field is too small for the shooting region (ipm.cuh:521-529)
 };
 
 int orc_version() { return 1; }
@@ -1119,25 +1124,61 @@ void orc_mass_moments(int mf, double ml, double mu, double ms, double* out3) { m
 
 double orc_mass_sample(int mf, double p, double ml, double mu, double ms) { return mass_sample(mf, p, ml, mu, ms); }
 
-// ipm.cuh:370-565 (star-file branch excluded: stars here always come from the generator)
-int orc_derived_params(const OrcInput* in, OrcDerived* d)
+static void star_params(int n, int rectangular, double c_re, double c_im, double theta, const double* stars, double* out);
+
+/* ipm.cuh:370-565. With file == nullptr the masses come from the mass
+   function and the star count and field from the sizing formulas; otherwise
+   from the star file (ipm.cuh:405-432, the moments of star.cuh:124-161 over
+   the file's stars and corner), and a circular field is turned towards the
+   shooting region's corner with the file's radius (ipm.cuh:507-529). */
+struct OrcStarFile
 {
-	const double kt = in->kappa_tot, g = in->shear, ks = in->kappa_star, th = in->theta_star;
-	if (in->mass_function == MF_EQUAL)
+	int n, rectangular;
+	double corner[2];
+	double theta;
+	const double* stars;
+};
+
+static void derive(const OrcInput* in, const OrcStarFile* file, OrcDerived* d)
+{
+	const double kt = in->kappa_tot, g = in->shear;
+	double ks = in->kappa_star, th = in->theta_star;
+	int rect = in->rectangular;
+	if (!file)
 	{
-		d->m_lower = 1;
-		d->m_upper = 1;
+		if (in->mass_function == MF_EQUAL)
+		{
+			d->m_lower = 1;
+			d->m_upper = 1;
+		}
+		else
+		{
+			d->m_lower = in->m_lower * in->m_solar;
+			d->m_upper = in->m_upper * in->m_solar;
+		}
+		double mm[3];
+		mass_moments(in->mass_function, d->m_lower, d->m_upper, in->m_solar, mm);
+		d->mean_mass = mm[0];
+		d->mean_mass2 = mm[1];
+		d->mean_mass2_ln_mass = mm[2];
 	}
 	else
 	{
-		d->m_lower = in->m_lower * in->m_solar;
-		d->m_upper = in->m_upper * in->m_solar;
+		double sp[6];
+		star_params(file->n, file->rectangular, file->corner[0], file->corner[1], file->theta, file->stars, sp);
+		rect = file->rectangular;
+		th = file->theta;
+		ks = sp[0];
+		d->m_lower = sp[1];
+		d->m_upper = sp[2];
+		d->mean_mass = sp[3];
+		d->mean_mass2 = sp[4];
+		d->mean_mass2_ln_mass = sp[5];
 	}
-	double mm[3];
-	mass_moments(in->mass_function, d->m_lower, d->m_upper, in->m_solar, mm);
-	d->mean_mass = mm[0];
-	d->mean_mass2 = mm[1];
-	d->mean_mass2_ln_mass = mm[2];
+	d->kappa_star = ks;
+	d->theta_star = th;
+	d->rectangular = rect;
+	d->covered = 1;
 	d->mu_ave = 1 / ((1 - kt) * (1 - kt) - g * g);
 	d->num_rays_x = 1.0 * in->num_rays_y * in->num_pixels_y[0] * in->num_pixels_y[1] / (2 * in->half_length_y[0] * 2 * in->half_length_y[1]);
 	double h1 = in->half_length_y[0], h2 = in->half_length_y[1];
@@ -1163,21 +1204,40 @@ int orc_derived_params(const OrcInput* in, OrcDerived* d)
 	double c1 = std::abs(d->center_x[0]) + d->half_length_x[0];
 	double c2 = std::abs(d->center_x[1]) + d->half_length_x[1];
 	double ss = in->safety_scale;
-	if (in->rectangular)
+	if (!file)
 	{
-		d->num_stars = int(std::ceil(ss * 2 * c1 * ss * 2 * c2 * ks / (PI * th * th * d->mean_mass)));
-		double a = std::sqrt(c1 / c2), b = std::sqrt(c2 / c1);
-		double f = std::sqrt(PI * th * th * d->num_stars * d->mean_mass / (4 * ks));
-		d->corner[0] = a * f;
-		d->corner[1] = b * f;
+		if (rect)
+		{
+			d->num_stars = int(std::ceil(ss * 2 * c1 * ss * 2 * c2 * ks / (PI * th * th * d->mean_mass)));
+			double a = std::sqrt(c1 / c2), b = std::sqrt(c2 / c1);
+			double f = std::sqrt(PI * th * th * d->num_stars * d->mean_mass / (4 * ks));
+			d->corner[0] = a * f;
+			d->corner[1] = b * f;
+		}
+		else
+		{
+			double ca = std::hypot(c1, c2);
+			d->num_stars = int(std::ceil(ss * ca * ss * ca * ks / (th * th * d->mean_mass)));
+			double f = std::sqrt(th * th * d->num_stars * d->mean_mass / ks);
+			d->corner[0] = c1 / ca * f;
+			d->corner[1] = c2 / ca * f;
+		}
 	}
 	else
 	{
-		double ca = std::hypot(c1, c2);
-		d->num_stars = int(std::ceil(ss * ca * ss * ca * ks / (th * th * d->mean_mass)));
-		double f = std::sqrt(th * th * d->num_stars * d->mean_mass / ks);
-		d->corner[0] = c1 / ca * f;
-		d->corner[1] = c2 / ca * f;
+		d->num_stars = file->n;
+		d->corner[0] = file->corner[0];
+		d->corner[1] = file->corner[1];
+		const double ca = std::hypot(c1, c2);
+		if (!rect)
+		{
+			const double r = std::hypot(file->corner[0], file->corner[1]);
+			d->corner[0] = c1 / ca * r;
+			d->corner[1] = c2 / ca * r;
+		}
+		if ((rect && (d->corner[0] < ss * c1 || d->corner[1] < ss * c2)) ||
+			(!rect && std::hypot(d->corner[0], d->corner[1]) < ss * ca))
+			d->covered = 0;
 	}
 	d->alpha_error = std::min(h1 / (10 * n1), h2 / (10 * n2));
 	int ts = 1;
@@ -1188,7 +1248,49 @@ int orc_derived_params(const OrcInput* in, OrcDerived* d)
 	while ((std::fmod(carg_ * (ts - 1), PI) < 0.1 * PI || std::fmod(carg_ * (ts - 1), PI) > 0.9 * PI) && ts <= 101)
 		ts += 2;
 	d->taylor_smooth = ts;
+}
+
+int orc_derived_params(const OrcInput* in, OrcDerived* d)
+{
+	derive(in, nullptr, d);
 	return 0;
+}
+
+/* the star-file branch: n stars (AoS x1, x2, m) with the file's field shape,
+   corner and theta; returns 0, or -1 when the field does not cover the
+   shooting region (the reference's run then fails, ipm.cuh:521-529) */
+int orc_derived_params_file(const OrcInput* in, int n, int rectangular, double c_re, double c_im, double theta,
+	const double* stars, OrcDerived* d)
+{
+	OrcStarFile f{n, rectangular, {c_re, c_im}, theta, stars};
+	derive(in, &f, d);
+	return d->covered ? 0 : -1;
+}
+
+/* the corner after the field is realised (ipm.cuh:717-731): the derived
+   corner's shape with the size set by the realised mean mass; the stars do
+   not move. out = [corner_re, corner_im, root_half_length (ipm.cuh:746-754)] */
+void orc_rescaled_corner(int rectangular, double c_re, double c_im, double theta, int n, double mean_mass_actual,
+	double kappa_star, double* out)
+{
+	double r, i;
+	if (rectangular)
+	{
+		const double f = std::sqrt(PI * theta * theta * n * mean_mass_actual / (4 * kappa_star));
+		r = std::sqrt(c_re / c_im) * f;
+		i = std::sqrt(c_im / c_re) * f;
+		out[2] = 1.1 * (r > i ? r : i);
+	}
+	else
+	{
+		const double a = std::hypot(c_re, c_im);
+		const double f = std::sqrt(theta * theta * n * mean_mass_actual / kappa_star);
+		r = c_re / a * f;
+		i = c_im / a * f;
+		out[2] = 1.1 * std::hypot(r, i);
+	}
+	out[0] = r;
+	out[1] = i;
 }
 
 // star.cuh:64-107 with the Philox stream above; out = AoS [x1, x2, m] * n
@@ -1221,6 +1323,11 @@ void orc_generate_stars(int n, int rectangular, double c_re, double c_im, int mf
 
 // star.cuh:124-161; out = [kappastar, m_lower, m_upper, mean_mass, mean_mass2, mean_mass2_ln_mass]
 void orc_star_params(int n, int rectangular, double c_re, double c_im, double theta, const double* stars, double* out)
+{
+	star_params(n, rectangular, c_re, c_im, theta, stars, out);
+}
+
+static void star_params(int n, int rectangular, double c_re, double c_im, double theta, const double* stars, double* out)
 {
 	double mlo = std::numeric_limits<double>::max();
 	double mhi = std::numeric_limits<double>::min();

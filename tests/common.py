@@ -63,29 +63,110 @@ def from_starfile(params, path):
     return kw
 
 
+def read_starfile(path):
+    """A star file in the reference's .bin layout (star.cuh:175-320), float or
+    double, told apart by its size: (n, rectangular, corner, theta, stars AoS)."""
+    import os
+    size = os.path.getsize(path)
+    with open(path, "rb") as fh:
+        n, rect = (int(x) for x in np.fromfile(fh, dtype=np.int32, count=2))
+        dt = np.float64 if size == 8 + 8 * 3 + 24 * n else np.float32
+        head = np.fromfile(fh, dtype=dt, count=3).astype(np.float64)
+        stars = np.fromfile(fh, dtype=dt, count=3 * n).astype(np.float64).reshape(n, 3)
+    return n, rect, (float(head[0]), float(head[1])), float(head[2]), stars
+
+
+def oracle_geometry(O, m):
+    """The oracle's derived parameters for an engine object's inputs: from its
+    star file when it has one (ipm.cuh:405-432), else from the mass function
+    (ipm.cuh:379-399). The engine scales m_lower and m_upper by m_solar in place
+    (ipm.cuh:381-390), so the inputs are recovered by the inverse scaling."""
+    y = dict(light_loss=m.light_loss, approx=int(m.approx), safety_scale=m.safety_scale,
+             center_y=(m.center_y1, m.center_y2), half_length_y=(m.half_length_y1, m.half_length_y2),
+             num_pixels_y=(m.num_pixels_y1, m.num_pixels_y2), num_rays_y=m.num_rays_y)
+    if m.starfile:
+        _, rect, corner, theta, stars = read_starfile(m.starfile)
+        d, covered = O.derived_params_file(stars, rect, corner, theta, m.kappa_tot, m.shear, **y)
+        assert covered
+        return d, (rect, corner, theta, stars)
+    mf = m.mass_function.lower()
+    ms = m.m_solar
+    ml, mu = (1.0, 1.0) if mf == "equal" else (m.m_lower / ms, m.m_upper / ms)
+    d = O.derived_params(m.kappa_tot, m.shear, m.kappa_star, m.theta_star, mf, ms, ml, mu,
+                         rectangular=int(m.rectangular), **y)
+    return d, None
+
+
 class Mirror:
-    """The oracle's view of a finished engine run: same stars, same derived
-    geometry, same lens parameters. `stars` defaults to the engine's own
-    (tree-ordered) stars as returned by the ABI."""
+    """The oracle's view of a finished engine run.
+
+    The oracle derives the geometry itself (oracle_geometry): the image-plane
+    region, the lattice, the star count, the smooth-term order, the star moments
+    of the realised field (star.cuh:124-161), the field corner re-scaled with
+    them (ipm.cuh:717-731) and the tree's root half-length (ipm.cuh:746-754).
+    Every one of the engine's derived getters is checked against it: to 1e-12
+    relative for the FP64 library, and to 3e-6 for a float library, whose
+    derivation runs in float as the reference's float build does. The oracle
+    then shoots with its own FP64 geometry; a float library's lattice, after
+    that check, is the engine's float one, the lattice that build shoots.
+
+    `stars` defaults to the engine's own (tree-ordered) stars from the ABI."""
 
     def __init__(self, O, m, p=None, stars=None):
         lib = m.lib
         obj = m.obj
         self.stars = np.array(m.stars if stars is None else stars, dtype=np.float64)
         self.p = m.expansion_order if p is None else p
-        corner_full = (lib.get_corner_x1(obj), lib.get_corner_x2(obj))
-        if not m.rectangular:
-            corner_full = (corner_full[0], 0.0)
-        self.lens = O.make_lens(m.kappa_tot, m.shear, m.theta_star, m.kappa_star, corner_full, m.rectangular,
-                                m.approx, lib.get_taylor_smooth(obj))
-        self.grid = O.Grid((ctypes.c_double * 2)(lib.get_center_x1(obj), lib.get_center_x2(obj)),
-                           (ctypes.c_double * 2)(lib.get_half_length_x1(obj), lib.get_half_length_x2(obj)),
-                           (ctypes.c_double * 2)(lib.get_ray_half_sep_1(obj), lib.get_ray_half_sep_2(obj)),
-                           (ctypes.c_int * 2)(lib.get_num_ray_threads_1(obj), lib.get_num_ray_threads_2(obj)),
-                           (ctypes.c_double * 2)(m.center_y1, m.center_y2),
-                           (ctypes.c_double * 2)(m.half_length_y1, m.half_length_y2),
-                           (ctypes.c_int * 2)(m.num_pixels_y1, m.num_pixels_y2))
-        self.root_half_length = float(lib.get_root_half_length(obj))
+        d, file = oracle_geometry(O, m)
+        self.derived = d
+        fp64 = bool(m.is_double)
+        rtol = 1e-12 if fp64 else 3e-6
+
+        def close(got, want, what):
+            np.testing.assert_allclose(got, want, rtol=rtol, atol=0, err_msg=what)
+
+        assert lib.get_num_stars(obj) == d.num_stars == len(self.stars)
+        assert (lib.get_num_ray_threads_1(obj), lib.get_num_ray_threads_2(obj)) == tuple(d.num_ray_threads)
+        assert lib.get_taylor_smooth(obj) == d.taylor_smooth
+        close([lib.get_center_x1(obj), lib.get_center_x2(obj)], list(d.center_x), "center_x")
+        close([lib.get_half_length_x1(obj), lib.get_half_length_x2(obj)], list(d.half_length_x), "half_length_x")
+        close([lib.get_ray_half_sep_1(obj), lib.get_ray_half_sep_2(obj)], list(d.ray_half_sep), "ray_half_sep")
+        close(lib.get_alpha_error(obj), d.alpha_error, "alpha_error")
+        close(m.kappa_star, d.kappa_star, "kappa_star")
+        # moments of the realised field, with the corner the reference takes
+        # them over: the derived one, or the file's
+        rect = d.rectangular
+        sp_corner = file[1] if file else tuple(d.corner)
+        engine_stars = np.array(m.stars, dtype=np.float64)
+        sp = O.star_params(engine_stars, rect, sp_corner, d.theta_star)
+        close([lib.get_kappa_star_actual(obj), lib.get_mean_mass_actual(obj), lib.get_mean_mass2_actual(obj),
+               lib.get_mean_mass2_ln_mass_actual(obj), lib.get_m_lower_actual(obj), lib.get_m_upper_actual(obj)],
+              [sp["kappa_star"], sp["mean_mass"], sp["mean_mass2"], sp["mean_mass2_ln_mass"], sp["m_lower"],
+               sp["m_upper"]], "star moments")
+        if file:
+            corner = tuple(d.corner)
+            root = 1.1 * (max(corner) if rect else float(np.hypot(*corner)))
+        else:
+            corner, root = O.rescaled_corner(rect, tuple(d.corner), d.theta_star, d.num_stars, sp["mean_mass"],
+                                             d.kappa_star)
+        want = corner if rect else (float(np.hypot(*corner)), 0.0)
+        close([lib.get_corner_x1(obj), lib.get_corner_x2(obj)], list(want), "corner")
+        close(lib.get_root_half_length(obj), root, "root_half_length")
+        self.root_half_length = root if fp64 else float(lib.get_root_half_length(obj))
+        corner_full = want if fp64 else ((lib.get_corner_x1(obj), lib.get_corner_x2(obj)) if rect
+                                         else (lib.get_corner_x1(obj), 0.0))
+        self.lens = O.make_lens(m.kappa_tot, m.shear, d.theta_star, d.kappa_star if fp64 else m.kappa_star, corner_full,
+                                rect, m.approx, d.taylor_smooth)
+        if fp64:
+            geo = (d.center_x, d.half_length_x, d.ray_half_sep)
+        else:
+            geo = ((lib.get_center_x1(obj), lib.get_center_x2(obj)),
+                   (lib.get_half_length_x1(obj), lib.get_half_length_x2(obj)),
+                   (lib.get_ray_half_sep_1(obj), lib.get_ray_half_sep_2(obj)))
+        D2, I2 = ctypes.c_double * 2, ctypes.c_int * 2
+        self.grid = O.Grid(D2(*geo[0]), D2(*geo[1]), D2(*geo[2]), I2(*d.num_ray_threads),
+                           D2(m.center_y1, m.center_y2), D2(m.half_length_y1, m.half_length_y2),
+                           I2(m.num_pixels_y1, m.num_pixels_y2))
         self.tree = O.Tree(self.stars, self.root_half_length)
         self.tree.fmm(self.p)
 

@@ -1,0 +1,177 @@
+"""The libraries' derived geometry against the oracle, on the CPU.
+
+ipm_derive / irs_derive run the reference's check_input_params and
+calculate_derived_params (ipm.cuh:235-565) without touching a device. Every
+derived getter of lib_ipm.so (float), lib_ipm_double.so and lib_irs.so is
+compared with oracle.derived_params (generated fields) or
+oracle.derived_params_file (star files) for the benchmark workloads, a few of
+the batched instances, rectangular fields, off-centre and unequal maps, and
+float and double star files.
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from common import C1_SMALL, C2_SMALL, C3_FULL, write_starfile
+
+C1_FULL = dict(C1_SMALL, num_pixels_y1=1000, num_pixels_y2=1000, num_rays_y=16, random_seed=0)
+C2_IPM = dict(C2_SMALL, half_length_y1=25.0, half_length_y2=25.0, num_pixels_y1=10000, num_pixels_y2=10000,
+              num_rays_y=10)
+C2_IRS = dict(C2_IPM, light_loss=0.01)
+RECT = dict(C2_SMALL, rectangular=True, approx=True, center_y1=0.7, center_y2=-0.4, half_length_y1=3.0,
+            half_length_y2=1.5, num_pixels_y1=120, num_pixels_y2=60)
+RECT_EXACT = dict(RECT, approx=False, mass_function="salpeter", m_lower=0.1, m_upper=1.0)
+OFF_CENTRE = dict(C1_SMALL, center_y1=-2.5, center_y2=4.0, half_length_y1=6.0, half_length_y2=2.0,
+                  num_pixels_y1=300, num_pixels_y2=100, mass_function="uniform", m_lower=0.2, m_upper=2.0,
+                  m_solar=1.5)
+OPTICAL = dict(C1_SMALL, mass_function="optical_depth", m_lower=0.3, m_upper=3.0)
+
+CASES = {"c1": C1_FULL, "c2_ipm": C2_IPM, "c2_irs": C2_IRS, "c3": C3_FULL, "rect_approx": RECT,
+         "rect_exact": RECT_EXACT, "off_centre": OFF_CENTRE, "optical_depth": OPTICAL}
+
+LIBS = [("float", False, False), ("double", True, False), ("irs", False, True)]
+
+
+def _engine(params, is_double, irs):
+    from irs_amd.microlensing.IPM.ipm import IPM
+    from irs_amd.microlensing.IRS.irs import IRS
+    m = IRS(**params) if irs else IPM(**params, is_double=is_double)
+    m.derive()
+    return m
+
+
+def _tol(is_double):
+    # float library: the reference's arithmetic in T = float
+    return 1e-13 if is_double else 3e-6
+
+
+def _oracle_for(oracle, p):
+    return oracle.derived_params(p["kappa_tot"], p["shear"], p["kappa_star"], p["theta_star"], p["mass_function"],
+                                 p["m_solar"], p["m_lower"], p["m_upper"], p["light_loss"], int(p["rectangular"]),
+                                 int(p["approx"]), p["safety_scale"], (p["center_y1"], p["center_y2"]),
+                                 (p["half_length_y1"], p["half_length_y2"]),
+                                 (p["num_pixels_y1"], p["num_pixels_y2"]), p["num_rays_y"])
+
+
+def check_geometry(m, d, rtol):
+    """Every derived getter of a derived (not yet run) engine object against an oracle record."""
+    lib, obj = m.lib, m.obj
+    close = lambda got, want, what: np.testing.assert_allclose(got, want, rtol=rtol, atol=0, err_msg=what)
+    assert lib.get_num_stars(obj) == d.num_stars
+    assert (lib.get_num_ray_threads_1(obj), lib.get_num_ray_threads_2(obj)) == tuple(d.num_ray_threads)
+    assert lib.get_taylor_smooth(obj) == d.taylor_smooth
+    close([lib.get_center_x1(obj), lib.get_center_x2(obj)], list(d.center_x), "center_x")
+    close([lib.get_half_length_x1(obj), lib.get_half_length_x2(obj)], list(d.half_length_x), "half_length_x")
+    close([lib.get_ray_half_sep_1(obj), lib.get_ray_half_sep_2(obj)], list(d.ray_half_sep), "ray_half_sep")
+    close(lib.get_alpha_error(obj), d.alpha_error, "alpha_error")
+    close(lib.get_mu_ave(obj), d.mu_ave, "mu_ave")
+    close([lib.get_mean_mass(obj), lib.get_mean_mass2(obj), lib.get_mean_mass2_ln_mass(obj)],
+          [d.mean_mass, d.mean_mass2, d.mean_mass2_ln_mass], "mass moments")
+    close([m.m_lower, m.m_upper], [d.m_lower, d.m_upper], "mass range")
+    close(m.kappa_star, d.kappa_star, "kappa_star")
+    close(m.theta_star, d.theta_star, "theta_star")
+    assert int(m.rectangular) == d.rectangular
+    # get_corner returns (|corner|, 0) for a circular field (ipm.cuh:1350)
+    want = tuple(d.corner) if d.rectangular else (math.hypot(*d.corner), 0.0)
+    close([lib.get_corner_x1(obj), lib.get_corner_x2(obj)], list(want), "corner")
+
+
+@pytest.mark.parametrize("lib", LIBS, ids=[x[0] for x in LIBS])
+@pytest.mark.parametrize("case", list(CASES))
+def test_derived_geometry_matches_oracle(oracle, lib, case):
+    _, is_double, irs = lib
+    p = CASES[case]
+    m = _engine(p, is_double, irs)
+    check_geometry(m, _oracle_for(oracle, p), _tol(is_double))
+    m.close()
+
+
+def test_batched_instances_match_oracle(oracle):
+    """Parameters of the heterogeneous batch (drawn on the host with the float library)."""
+    from irs_amd.microlensing.IPM.batch import draw_instances
+    base = dict(theta_star=1.0, m_solar=1.0, light_loss=0.001, rectangular=False, approx=True, safety_scale=1.37,
+                center_y1=0.0, center_y2=0.0)
+    for inst in draw_instances(6, seed=11):
+        p = dict(base, **inst)
+        for is_double in (False, True):
+            m = _engine(p, is_double, False)
+            check_geometry(m, _oracle_for(oracle, p), _tol(is_double))
+            m.close()
+
+
+def _write_starfile(path, stars, rectangular, corner, theta, dtype):
+    # the reference's layouts (star.cuh:237-271): corner, theta and stars in float or double
+    with open(path, "wb") as fh:
+        np.array([len(stars), int(rectangular)], dtype=np.int32).tofile(fh)
+        np.array([corner[0], corner[1], theta], dtype=dtype).tofile(fh)
+        stars.astype(dtype).tofile(fh)
+
+
+def _read_header(path):
+    with open(path, "rb") as fh:
+        n, rect = np.fromfile(fh, dtype=np.int32, count=2)
+        c = np.fromfile(fh, dtype=np.float64, count=3)
+    return int(n), int(rect), (float(c[0]), float(c[1])), float(c[2])
+
+
+STARFILE_CASES = {"circle": dict(C1_SMALL), "rectangle": dict(C2_SMALL, rectangular=True),
+                  "off_centre_kroupa": dict(C2_SMALL, center_y1=0.5, center_y2=0.25, half_length_y1=1.0)}
+
+
+@pytest.mark.parametrize("lib", LIBS, ids=[x[0] for x in LIBS])
+@pytest.mark.parametrize("case", list(STARFILE_CASES))
+@pytest.mark.parametrize("fmt", ["double", "float"])
+def test_star_file_geometry_matches_oracle(oracle, tmp_path, lib, case, fmt):
+    from common import from_starfile
+    _, is_double, irs = lib
+    p = STARFILE_CASES[case]
+    path = os.path.join(tmp_path, "stars.bin")
+    stars = write_starfile(oracle, path, p, seed=3)
+    n, rect, corner, theta = _read_header(path)
+    # a field 5% larger than the one drawn, so that coverage (ipm.cuh:521-529)
+    # is not decided by the last bit
+    corner = (1.05 * corner[0], 1.05 * corner[1])
+    dt = np.float32 if fmt == "float" else np.float64
+    _write_starfile(path, stars, rect, corner, theta, dt)
+    stars = stars.astype(dt).astype(np.float64)
+    corner = tuple(float(x) for x in np.asarray(corner, dtype=dt))
+    theta = float(dt(theta))
+    kw = from_starfile(p, path)
+    m = _engine(kw, is_double, irs)
+    d, covered = oracle.derived_params_file(stars, rect, corner, theta, p["kappa_tot"], p["shear"], p["light_loss"],
+                                            int(p["approx"]), p["safety_scale"], (p["center_y1"], p["center_y2"]),
+                                            (p["half_length_y1"], p["half_length_y2"]),
+                                            (p["num_pixels_y1"], p["num_pixels_y2"]), p["num_rays_y"])
+    assert covered
+    # the star moments of the file are those of the realised field (star.cuh:124-161)
+    sp = oracle.star_params(stars, rect, corner, theta)
+    np.testing.assert_allclose([m.lib.get_kappa_star_actual(m.obj), m.lib.get_mean_mass_actual(m.obj),
+                                m.lib.get_mean_mass2_actual(m.obj), m.lib.get_mean_mass2_ln_mass_actual(m.obj),
+                                m.lib.get_m_lower_actual(m.obj), m.lib.get_m_upper_actual(m.obj)],
+                               [sp["kappa_star"], sp["mean_mass"], sp["mean_mass2"], sp["mean_mass2_ln_mass"],
+                                sp["m_lower"], sp["m_upper"]], rtol=_tol(is_double) * 10)
+    check_geometry(m, d, _tol(is_double) * 10)
+    m.close()
+
+
+def test_star_file_too_small_is_refused(oracle, tmp_path):
+    from common import from_starfile
+    from irs_amd.microlensing.IPM.ipm import IPM
+    p = dict(C1_SMALL)
+    path = os.path.join(tmp_path, "stars.bin")
+    write_starfile(oracle, path, p, seed=3)
+    # the same file, asked to cover a region twice as large
+    kw = from_starfile(dict(p, half_length_y1=20.0, half_length_y2=20.0), path)
+    m = IPM(**kw, is_double=True)
+    with pytest.raises(Exception):
+        m.derive()
+    n, rect, corner, theta = _read_header(path)
+    stars = np.fromfile(path, dtype=np.float64, offset=32).reshape(-1, 3)
+    _, covered = oracle.derived_params_file(stars, rect, corner, theta, p["kappa_tot"], p["shear"], p["light_loss"],
+                                            1, p["safety_scale"], (0.0, 0.0), (20.0, 20.0),
+                                            (p["num_pixels_y1"], p["num_pixels_y2"]), p["num_rays_y"])
+    assert not covered
+    m.close()

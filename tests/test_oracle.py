@@ -150,6 +150,116 @@ def test_c2_star_count(oracle):
     assert abs(d.num_stars - 0.36 * R ** 2 / d.mean_mass) <= 1
 
 
+def test_expansion_order_closed_form(oracle):
+    """ipm.cuh:848-860, p = ceil(2 log2(theta) + L - log2(root_hl) - log2(alpha_err)).
+    Powers of two make every term an exact integer, so each case below moves one
+    term at a time by a known step: a dropped factor 2, a flipped sign or a swapped
+    argument changes at least one of them."""
+    eo = oracle.expansion_order
+    assert eo(1.0, 6, 64.0, 2.0 ** -10) == 10          # 0 + 6 - 6 + 10
+    assert eo(2.0, 6, 64.0, 2.0 ** -10) == 12          # theta x2: +2 (the factor 2 on log2 theta)
+    assert eo(0.5, 6, 64.0, 2.0 ** -10) == 8           # theta /2: -2
+    assert eo(1.0, 9, 64.0, 2.0 ** -10) == 13          # three more levels: +3
+    assert eo(1.0, 6, 256.0, 2.0 ** -10) == 8          # root 4x larger: -2
+    assert eo(1.0, 6, 64.0, 2.0 ** -13) == 13          # error 8x smaller: +3
+    assert eo(1.0, 6, 64.0, 2.0 ** -10 * 1.01) == 10   # ceil(9.9856...) = 10
+    assert eo(1.0, 6, 64.0 * 1.01, 2.0 ** -10) == 10   # ceil(9.9856...) = 10
+    assert eo(1.0, 6, 64.0, 2.0 ** -10 / 1.01) == 11   # ceil(10.0144...) = 11
+    # the workloads, worked out by hand with log2(x) = ln(x) / ln(2):
+    # small FP64 case, root_hl = 1.1 * 100.8 = 110.9: 6 - 6.7931 + 9.9658 = 9.17 -> 10
+    assert eo(1.0, 6, 110.9, 1e-3) == 10
+    # 10000^2 case: field radius 1076.26, root_hl = 1.1 * 1076.26 = 1183.9, alpha = 25 / 1e5:
+    # 10 - 10.209 + 11.966 = 11.76 -> 12
+    assert eo(1.0, 10, 1183.9, 2.5e-4) == 12
+    # FMM stress case: radius 959.31, root_hl = 1055.2, alpha = 50 / (10 * 20000) = 2.5e-4, 11 levels:
+    # 11 - 10.043 + 11.966 = 12.92 -> 13
+    assert eo(1.0, 11, 1055.2, 2.5e-4) == 13
+
+
+def test_star_params_closed_form(oracle):
+    """star.cuh:124-161 over a fixture list whose moments are exact by hand."""
+    stars = np.array([[0.3, -1.0, 1.0], [2.0, 0.5, 2.0], [-4.0, 1.5, 4.0]])
+    ln2 = math.log(2.0)
+    # circle of radius |(3, 4)| = 5, theta = 2: kappa = m_tot theta^2 / R^2 = 7 * 4 / 25
+    sp = oracle.star_params(stars, 0, (3.0, 4.0), 2.0)
+    assert sp["kappa_star"] == pytest.approx(28 / 25, rel=1e-15)
+    assert (sp["m_lower"], sp["m_upper"]) == (1.0, 4.0)
+    assert sp["mean_mass"] == pytest.approx(7 / 3, rel=1e-15)
+    assert sp["mean_mass2"] == pytest.approx(21 / 3, rel=1e-15)
+    # (1 ln 1 + 4 ln 2 + 16 ln 4) / 3 = 36 ln 2 / 3
+    assert sp["mean_mass2_ln_mass"] == pytest.approx(12 * ln2, rel=1e-15)
+    # rectangle with corner (2, 3), theta = 1: kappa = m_tot pi theta^2 / (4 c1 c2) = 7 pi / 24
+    sp = oracle.star_params(stars, 1, (2.0, 3.0), 1.0)
+    assert sp["kappa_star"] == pytest.approx(7 * math.pi / 24, rel=1e-15)
+    # one mass only: m_lower = m_upper and <m^2 ln m> = m^2 ln m
+    sp = oracle.star_params(np.tile([[1.0, 1.0, 0.5]], (10, 1)), 0, (1.0, 0.0), 1.0)
+    assert (sp["m_lower"], sp["m_upper"]) == (0.5, 0.5)
+    assert sp["mean_mass2_ln_mass"] == pytest.approx(-0.25 * ln2, rel=1e-15)
+    assert sp["kappa_star"] == pytest.approx(5.0, rel=1e-15)
+
+
+def test_rescaled_corner_closed_form(oracle):
+    """ipm.cuh:717-731 and the tree root ipm.cuh:746-754."""
+    # circle: direction (0.6, 0.8) kept, radius sqrt(theta^2 N <m> / kappa) = sqrt(100 * 0.25 / 0.5)
+    c, root = oracle.rescaled_corner(0, (3.0, 4.0), 1.0, 100, 0.25, 0.5)
+    np.testing.assert_allclose(c, (0.6 * math.sqrt(50), 0.8 * math.sqrt(50)), rtol=1e-15)
+    assert root == pytest.approx(1.1 * math.sqrt(50), rel=1e-15)
+    # rectangle (2, 8): shape (sqrt(1/4), sqrt(4)) times sqrt(pi theta^2 N <m> / (4 kappa)) with theta = 2
+    c, root = oracle.rescaled_corner(1, (2.0, 8.0), 2.0, 100, 0.25, 0.5)
+    f = math.sqrt(math.pi * 4 * 100 * 0.25 / 2.0)
+    np.testing.assert_allclose(c, (0.5 * f, 2.0 * f), rtol=1e-15)
+    assert root == pytest.approx(1.1 * 2.0 * f, rel=1e-15)
+    # with the model's own mean mass the derived corner comes back unchanged
+    for rect in (0, 1):
+        d = oracle.derived_params(0.45, 0.5, 0.36, 1.0, "kroupa", 1.0, 0.08, 1.5, 0.001, rect, 1, 1.37, (0, 0),
+                                  (25, 25), (10000, 10000), 10)
+        c, _ = oracle.rescaled_corner(rect, tuple(d.corner), 1.0, d.num_stars, d.mean_mass, 0.36)
+        np.testing.assert_allclose(c, tuple(d.corner), rtol=1e-14)
+
+
+def test_star_file_branch_of_derived_params(oracle):
+    """ipm.cuh:405-432 and 507-529: a field read from a file sets the star count,
+    theta and the masses; a circular field is turned to the shooting region's
+    corner with the file's radius, and too small a field is refused."""
+    gen = oracle.derived_params(0.3, 0.3, 0.3, 1.0, "equal", 1.0, 1.0, 1.0, 0.001, 0, 1, 1.37, (0, 0), (10, 10),
+                                (1000, 1000), 16)
+    n = gen.num_stars
+    R = math.hypot(*gen.corner)
+    # unit masses in a disk of the generated radius: the file's kappa_star is
+    # N / R^2 = 0.3 by construction, so the geometry must equal the generated one
+    rng = np.random.default_rng(0)
+    stars = np.column_stack([rng.uniform(-R / 2, R / 2, n), rng.uniform(-R / 2, R / 2, n), np.ones(n)])
+    d, ok = oracle.derived_params_file(stars, 0, (R, 0.0), 1.0, 0.3, 0.3, 0.001, 1, 1.37, (0, 0), (10, 10),
+                                       (1000, 1000), 16)
+    assert ok and d.num_stars == n and d.rectangular == 0
+    assert d.kappa_star == pytest.approx(0.3, rel=1e-12)
+    np.testing.assert_allclose(d.half_length_x, gen.half_length_x, rtol=1e-12)
+    assert tuple(d.num_ray_threads) == tuple(gen.num_ray_threads)
+    np.testing.assert_allclose(d.corner, gen.corner, rtol=1e-12)   # (R, 0) turned to the region's corner
+    assert d.taylor_smooth == gen.taylor_smooth
+    # theta and masses come from the file: theta 2 with masses 1/4 keeps kappa
+    s2 = stars.copy()
+    s2[:, 2] = 0.25
+    d2, ok = oracle.derived_params_file(s2, 0, (R, 0.0), 2.0, 0.3, 0.3, 0.001, 1, 1.37, (0, 0), (10, 10),
+                                        (1000, 1000), 16)
+    assert ok and d2.theta_star == 2.0 and d2.mean_mass == 0.25 and d2.kappa_star == pytest.approx(0.3, rel=1e-12)
+    # a field 10% below the safety-scaled radius is refused
+    need = 1.37 * math.hypot(gen.half_length_x[0], gen.half_length_x[1])
+    _, ok = oracle.derived_params_file(stars, 0, (0.9 * need, 0.0), 1.0, 0.3, 0.3, 0.001, 1, 1.37, (0, 0), (10, 10),
+                                       (1000, 1000), 16)
+    assert not ok
+    # a rectangle keeps its corner (its kappa_star, and so the region, follow
+    # from the corner) and needs each side to cover
+    c = (3.0 * gen.half_length_x[0], 3.0 * gen.half_length_x[1])
+    d3, ok = oracle.derived_params_file(stars, 1, c, 1.0, 0.3, 0.3, 0.001, 1, 1.37, (0, 0), (10, 10), (1000, 1000),
+                                        16)
+    assert ok and tuple(d3.corner) == c and d3.kappa_star == pytest.approx(n * math.pi / (4 * c[0] * c[1]))
+    assert 1.37 * d3.half_length_x[0] <= c[0] and 1.37 * d3.half_length_x[1] <= c[1]
+    _, ok = oracle.derived_params_file(stars, 1, (c[0], c[1] / 6), 1.0, 0.3, 0.3, 0.001, 1, 1.37, (0, 0), (10, 10),
+                                       (1000, 1000), 16)
+    assert not ok
+
+
 # --------------------------------------------------------------------- deflection
 def lens_for(oracle, kappa, gamma, ks, R, approx=1):
     return oracle.make_lens(kappa, gamma, 1.0, ks, (R, 0.0), 0, approx, 1)
