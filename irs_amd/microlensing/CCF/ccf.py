@@ -5,6 +5,7 @@ arguments, property validation, run() / save() and plots."""
 import numpy as np
 
 from . import lib_ccf
+from .. import _plotting
 
 MASS_FUNCTIONS = ("equal", "uniform", "salpeter", "kroupa", "optical_depth")
 
@@ -280,20 +281,24 @@ class CCF(object):
     def num_roots(self):
         return self.lib.get_num_roots(self.obj)
 
+    def _copy(self, getter, shape, wanted=True):
+        """A copy of one of the library's result arrays, or None when that
+        output was not asked for."""
+        if not wanted:
+            return None
+        return np.array(np.ctypeslib.as_array(getattr(self.lib, getter)(self.obj), shape=shape))
+
     def run(self):
+        """Find the critical curves (and, as asked, the caustics and the
+        magnification length scales). Curves are (roots x branches) rows of
+        num_phi / num_branches + 1 points, as (re, im) pairs."""
         if not self.lib.run(self.obj, self.verbose):
             raise Exception("Error running CCF")
-        shape = (self.num_roots * self.num_branches, self.num_phi // self.num_branches + 1)
-        self.critical_curves = np.ctypeslib.as_array(self.lib.get_critical_curves(self.obj), shape=shape + (2,)).copy()
-        if self.write_caustics:
-            self.caustics = np.ctypeslib.as_array(self.lib.get_caustics(self.obj), shape=shape + (2,)).copy()
-        else:
-            self.caustics = None
-        if self.write_mu_length_scales:
-            self.mu_length_scales = np.ctypeslib.as_array(self.lib.get_mu_length_scales(self.obj), shape=shape).copy()
-        else:
-            self.mu_length_scales = None
-        self.stars = np.ctypeslib.as_array(self.lib.get_stars(self.obj), shape=(self.num_stars, 3)).copy()
+        curves = (self.num_roots * self.num_branches, self.num_phi // self.num_branches + 1)
+        self.critical_curves = self._copy("get_critical_curves", curves + (2,))
+        self.caustics = self._copy("get_caustics", curves + (2,), self.write_caustics)
+        self.mu_length_scales = self._copy("get_mu_length_scales", curves, self.write_mu_length_scales)
+        self.stars = self._copy("get_stars", (self.num_stars, 3))
 
     @property
     def t_ccs(self):
@@ -312,11 +317,7 @@ class CCF(object):
             raise Exception("Error saving CCF")
 
     def plot_critical_curves(self, fig, ax, **kwargs):
-        ax.plot(self.critical_curves[:, :, 0].T, self.critical_curves[:, :, 1].T, **kwargs)
-        ax.set_xlabel('$x_1 / \\theta_★$')
-        ax.set_ylabel('$x_2 / \\theta_★$')
+        _plotting.curves(ax, self.critical_curves, _plotting.X_LABELS, kwargs)
 
     def plot_caustics(self, fig, ax, **kwargs):
-        ax.plot(self.caustics[:, :, 0].T, self.caustics[:, :, 1].T, **kwargs)
-        ax.set_xlabel('$y_1 / \\theta_★$')
-        ax.set_ylabel('$y_2 / \\theta_★$')
+        _plotting.curves(ax, self.caustics, _plotting.Y_LABELS, kwargs)

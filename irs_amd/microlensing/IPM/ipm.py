@@ -19,6 +19,7 @@ import numpy as np
 
 from . import lib_ipm
 from . import lib_ipm_double
+from .. import _plotting
 
 
 class IPM(object):
@@ -611,28 +612,20 @@ class IPM(object):
             raise Exception("Error saving IPM")
 
     def plot_map(self, fig, ax, **kwargs):
-        if "vmin" not in kwargs:
-            kwargs["vmin"] = np.percentile(self.magnitudes.ravel(), 2.5)
-        if "vmax" not in kwargs:
-            kwargs["vmax"] = np.percentile(self.magnitudes.ravel(), 97.5)
-        if "cmap" not in kwargs:
-            kwargs["cmap"] = "viridis_r"
-        extent = [(self.center[0] - self.half_length[0]) / self.theta_star,
-                  (self.center[0] + self.half_length[0]) / self.theta_star,
-                  (self.center[1] - self.half_length[1]) / self.theta_star,
-                  (self.center[1] + self.half_length[1]) / self.theta_star]
-        img = ax.imshow(self.magnitudes, extent=extent, **kwargs)
-        cbar = fig.colorbar(img, label="microlensing $\\Delta m$ (magnitudes)")
-        cbar.ax.invert_yaxis()
-        ax.set_xlabel("$y_1 / \\theta_★$")
-        ax.set_ylabel("$y_2 / \\theta_★$")
-        ax.set_aspect(self.half_length[0] / self.half_length[1])
+        """The map in magnitudes relative to |mu_ave|, bright pixels dark; the
+        default colour range spans the central 95% of the pixels."""
+        dm = self.magnitudes
+        _plotting.image(fig, ax, dm, _plotting.box(self.center, self.half_length, self.theta_star),
+                        self.half_length[0] / self.half_length[1], "microlensing $\\Delta m$ (magnitudes)",
+                        _plotting.Y_LABELS, kwargs,
+                        {"vmin": np.percentile(dm, 2.5), "vmax": np.percentile(dm, 97.5), "cmap": "viridis_r"},
+                        flip_colorbar=True)
 
     def plot_hist(self, fig, ax, bins=None, **kwargs):
+        """Density of the pixels' magnitudes in bins of 0.01 mag, brightest at
+        the right."""
+        dm = self.magnitudes
         if bins is None:
-            vmin, vmax = (np.min(self.magnitudes), np.max(self.magnitudes))
-            bins = np.arange(vmin - 0.01, vmax + 0.01, 0.01)
-        ax.hist(self.magnitudes.ravel(), bins=bins, density=True, **kwargs)
-        ax.set_xlabel("microlensing $\\Delta m$ (magnitudes)")
-        ax.set_ylabel("p($\\Delta m$)")
-        ax.invert_xaxis()
+            bins = np.arange(np.min(dm) - 0.01, np.max(dm) + 0.01, 0.01)
+        _plotting.density(ax, dm, bins, "microlensing $\\Delta m$ (magnitudes)", "p($\\Delta m$)", kwargs,
+                          flip_x=True)
