@@ -2,7 +2,7 @@
 
 CPU FP64 restatement of the histogram surrogate network of
 irs_amd/csrc/surrogate.hip, the checker for its GPU forward and backward
-passes. Network: 6 inputs zero-padded to 16 -> 256 -> 256 -> 256 -> 128,
+passes and its Adam step. Network: 6 inputs zero-padded to 16 -> 256 -> 256 -> 256 -> 128,
 SiLU hidden activations, softmax output, loss = mean over rows of
 -sum_j t_j log p_j. Parameters are flat, layer by layer: W [out x in]
 row-major, then b [out].
@@ -15,7 +15,12 @@ With bf16=True the operands of every matrix product are rounded to BF16
 The bias gradients use the unrounded dZ, as on the GPU.
 
 The backward pass is pinned by central finite differences of the loss in
-FP64 (tests/test_surrogate.py).
+FP64 (tests/test_surrogate.py). The Adam step is pinned by closed forms of
+the published algorithm: the first step moves every weight with a non-zero
+gradient by lr against the gradient's sign; a constant gradient moves it by
+lr g / (|g| + eps) on every step (the bias corrections make the moment
+estimates exact); with a zero gradient the weights shrink by (1 - lr wd) per
+step and the biases stay.
 """
 
 import numpy as np
@@ -113,3 +118,31 @@ def gradients(params, x, t, bf16_ops=False):
         dh = dr @ r(Ws[l])
         d = dh * dsilu(zs[l - 1])
     return L, g
+
+
+def decay_mask():
+    """1 for the weight-matrix entries, 0 for the biases: the GPU applies the
+    decoupled weight decay to the weight matrices only."""
+    offs, n = layout()
+    mask = np.zeros(n)
+    for l, (w, b) in enumerate(offs):
+        mask[w:b] = 1.0
+    return mask
+
+
+def adam_step(params, grads, m, v, step, lr, wd, b1=0.9, b2=0.999, eps=1e-8):
+    """One step of Adam (Kingma & Ba 2015, Algorithm 1) with decoupled weight
+    decay (Loshchilov & Hutter 2019, AdamW) in FP64. `step` counts from 1.
+    Returns (params, m, v); the inputs are not modified.
+
+        m <- b1 m + (1 - b1) g,   v <- b2 v + (1 - b2) g^2
+        w <- w (1 - lr wd [weights only]) - lr (m / (1 - b1^step)) / (sqrt(v / (1 - b2^step)) + eps)
+    """
+    g = np.asarray(grads, dtype=np.float64)
+    m = b1 * np.asarray(m, dtype=np.float64) + (1.0 - b1) * g
+    v = b2 * np.asarray(v, dtype=np.float64) + (1.0 - b2) * g * g
+    mhat = m / (1.0 - b1 ** step)
+    vhat = v / (1.0 - b2 ** step)
+    w = np.asarray(params, dtype=np.float64) * (1.0 - lr * wd * decay_mask())
+    w = w - lr * mhat / (np.sqrt(vhat) + eps)
+    return w, m, v

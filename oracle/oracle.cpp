@@ -105,12 +105,14 @@ salpeter.cuh:22, uniform.cuh, equal.cuh:27-72, optical_depth.cuh:28-91.
 --------------------------------------------------------------------------- */
 enum MassFunctionId { MF_EQUAL = 0, MF_UNIFORM = 1, MF_SALPETER = 2, MF_KROUPA = 3, MF_OPTICAL_DEPTH = 4 };
 
-double power_integral(double x1, double x2, double a, double b = 1)           // power_law.cuh:30-44
+template <typename T>
+T power_integral(T x1, T x2, T a, T b = 1)                                      // power_law.cuh:30-44
 {
 	if (a != -1) return b * (std::pow(x2, a + 1) - std::pow(x1, a + 1)) / (a + 1);
 	return b * (std::log(x2) - std::log(x1));
 }
-double power_log_integral(double x1, double x2, double a, double b = 1)       // power_law.cuh:59-73
+template <typename T>
+T power_log_integral(T x1, T x2, T a, T b = 1)                                  // power_law.cuh:59-73
 {
 	if (a != -1)
 		return b * (std::pow(x2, a + 1) * ((a + 1) * std::log(x2) - 1) - std::pow(x1, a + 1) * ((a + 1) * std::log(x1) - 1)) / ((a + 1) * (a + 1));
@@ -129,28 +131,31 @@ continuous across the breaks. The reference enumerates the same segment
 combinations branch by branch (kroupa.cuh:63-123); a segment is present iff
 it has positive width, which is exactly the reference's strict comparisons.
 */
-struct Segments
+template <typename T>
+struct SegmentsT
 {
 	int n = 0;
-	double lo[3], hi[3], slope[3], b[3], prob[3];
+	T lo[3], hi[3], slope[3], b[3], prob[3];
 };
+using Segments = SegmentsT<double>;
 
-Segments make_segments(int mf, double ml, double mu, double ms)
+template <typename T>
+SegmentsT<T> make_segments(int mf, T ml, T mu, T ms)
 {
-	Segments s;
+	SegmentsT<T> s;
 	if (mf == MF_KROUPA)
 	{
-		const double slopes[3] = {-0.3, -1.3, -2.3};
-		const double m0 = 0.08 * ms;
-		const double m1 = 0.5 * ms;
-		const double edges[4] = {0, m0, m1, std::numeric_limits<double>::infinity()};
-		double k = 1;        // continuity factor relative to the first present segment
+		const T slopes[3] = {T(-0.3), T(-1.3), T(-2.3)};
+		const T m0 = T(0.08) * ms;
+		const T m1 = T(0.5) * ms;
+		const T edges[4] = {0, m0, m1, std::numeric_limits<T>::infinity()};
+		T k = 1;        // continuity factor relative to the first present segment
 		bool first = true;
-		double total = 0;
+		T total = 0;
 		for (int i = 0; i < 3; i++)
 		{
-			double lo = std::max(ml, edges[i]);
-			double hi = std::min(mu, edges[i + 1]);
+			T lo = std::max(ml, edges[i]);
+			T hi = std::min(mu, edges[i + 1]);
 			if (!(lo < hi)) continue;
 			if (!first) k *= std::pow(edges[i], slopes[i - 1] - slopes[i]);
 			first = false;
@@ -158,7 +163,7 @@ Segments make_segments(int mf, double ml, double mu, double ms)
 			s.hi[s.n] = hi;
 			s.slope[s.n] = slopes[i];
 			s.b[s.n] = k;
-			s.prob[s.n] = power_integral(lo, hi, slopes[i]) * k;
+			s.prob[s.n] = power_integral<T>(lo, hi, slopes[i]) * k;
 			total += s.prob[s.n];
 			s.n++;
 		}
@@ -170,12 +175,12 @@ Segments make_segments(int mf, double ml, double mu, double ms)
 	}
 	else
 	{
-		double slope = (mf == MF_SALPETER) ? -2.35 : 0.0;
+		T slope = (mf == MF_SALPETER) ? T(-2.35) : T(0);
 		s.n = 1;
 		s.lo[0] = ml;
 		s.hi[0] = mu;
 		s.slope[0] = slope;
-		s.b[0] = 1 / power_integral(ml, mu, slope);
+		s.b[0] = 1 / power_integral<T>(ml, mu, slope);
 		s.prob[0] = 1;
 	}
 	return s;
@@ -193,7 +198,7 @@ double mass_sample(int mf, double p, double ml, double mu, double ms)
 	default:
 	{
 		if (ml == mu) return ml;                                                  // power_law.cuh:128-131
-		Segments s = make_segments(mf, ml, mu, ms);
+		Segments s = make_segments<double>(mf, ml, mu, ms);
 		double cum = 0;
 		for (int i = 0; i < s.n; i++)
 		{
@@ -208,7 +213,10 @@ double mass_sample(int mf, double p, double ml, double mu, double ms)
 	}
 }
 
-void mass_moments(int mf, double ml, double mu, double ms, double out[3])
+/* the moments <m>, <m^2>, <m^2 ln m> in the value type T, as the reference's
+   mass functions evaluate them (mass_function<T>) */
+template <typename T>
+void mass_moments(int mf, T ml, T mu, T ms, T out[3])
 {
 	switch (mf)
 	{
@@ -238,13 +246,13 @@ void mass_moments(int mf, double ml, double mu, double ms, double out[3])
 			out[2] = ml * ml * std::log(ml);
 			return;
 		}
-		Segments s = make_segments(mf, ml, mu, ms);
+		SegmentsT<T> s = make_segments<T>(mf, ml, mu, ms);
 		out[0] = out[1] = out[2] = 0;
 		for (int i = 0; i < s.n; i++)
 		{
-			out[0] += power_integral(s.lo[i], s.hi[i], s.slope[i] + 1, s.b[i]);
-			out[1] += power_integral(s.lo[i], s.hi[i], s.slope[i] + 2, s.b[i]);
-			out[2] += power_log_integral(s.lo[i], s.hi[i], s.slope[i] + 2, s.b[i]);
+			out[0] += power_integral<T>(s.lo[i], s.hi[i], s.slope[i] + 1, s.b[i]);
+			out[1] += power_integral<T>(s.lo[i], s.hi[i], s.slope[i] + 2, s.b[i]);
+			out[2] += power_log_integral<T>(s.lo[i], s.hi[i], s.slope[i] + 2, s.b[i]);
 		}
 	}
 	}
@@ -1120,7 +1128,7 @@ void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) 
 
 void orc_star_uniforms(uint32_t seed, int64_t i, double* u4) { star_uniforms(seed, i, u4); }
 
-void orc_mass_moments(int mf, double ml, double mu, double ms, double* out3) { mass_moments(mf, ml, mu, ms, out3); }
+void orc_mass_moments(int mf, double ml, double mu, double ms, double* out3) { mass_moments<double>(mf, ml, mu, ms, out3); }
 
 double orc_mass_sample(int mf, double p, double ml, double mu, double ms) { return mass_sample(mf, p, ml, mu, ms); }
 
@@ -1139,158 +1147,240 @@ struct OrcStarFile
 	const double* stars;
 };
 
+/* The reference derives in its value type T (IPM<T>, ipm.cuh:370-565):
+   lib_ipm.so in float, lib_ipm_double.so in double. derive<T> evaluates each
+   formula in T with the reference's operand types (its inputs and members are
+   T, the literals 1.0 and the pixel counts as written there), so a float
+   evaluation lands on the same side of each ceil and floor as the float
+   build. The mass moments come from mass_moments<T>; the moments of a star
+   file accumulate in double over T masses as in star.cuh:124-161. Results are stored widened to double. */
+}  // extern "C"
+
+template <typename T>
+static void star_params_t(int n, int rectangular, T c_re, T c_im, T theta, const double* stars, T* out)
+{
+	T mlo = std::numeric_limits<T>::max();
+	T mhi = std::numeric_limits<T>::min();
+	double mt = 0, m2 = 0, m2l = 0;
+	for (int i = 0; i < n; i++)
+	{
+		T m = T(stars[3 * i + 2]);
+		mt += m;
+		m2 += m * m;
+		m2l += m * m * std::log(m);
+		mlo = std::min(mlo, m);
+		mhi = std::max(mhi, m);
+	}
+	const T pi = T(PI);
+	if (rectangular)
+		out[0] = T(mt * pi * theta * theta / (4 * c_re * c_im));
+	else
+		out[0] = T(mt * theta * theta / (std::hypot(c_re, c_im) * std::hypot(c_re, c_im)));
+	out[1] = mlo;
+	out[2] = mhi;
+	out[3] = T(mt / n);
+	out[4] = T(m2 / n);
+	out[5] = T(m2l / n);
+}
+
+template <typename T>
 static void derive(const OrcInput* in, const OrcStarFile* file, OrcDerived* d)
 {
-	const double kt = in->kappa_tot, g = in->shear;
-	double ks = in->kappa_star, th = in->theta_star;
+	const T kt = T(in->kappa_tot), g = T(in->shear);
+	T ks = T(in->kappa_star), th = T(in->theta_star);
+	const T ll = T(in->light_loss), ss = T(in->safety_scale);
+	const T pi = T(PI), inv_pi = T(1 / PI);
 	int rect = in->rectangular;
+	T mean_mass, mean_mass2;
 	if (!file)
 	{
-		if (in->mass_function == MF_EQUAL)
+		T ml = 1, mu = 1;
+		if (in->mass_function != MF_EQUAL)
 		{
-			d->m_lower = 1;
-			d->m_upper = 1;
+			ml = T(in->m_lower) * T(in->m_solar);
+			mu = T(in->m_upper) * T(in->m_solar);
 		}
-		else
-		{
-			d->m_lower = in->m_lower * in->m_solar;
-			d->m_upper = in->m_upper * in->m_solar;
-		}
-		double mm[3];
-		mass_moments(in->mass_function, d->m_lower, d->m_upper, in->m_solar, mm);
-		d->mean_mass = mm[0];
-		d->mean_mass2 = mm[1];
+		d->m_lower = ml;
+		d->m_upper = mu;
+		T mm[3];
+		mass_moments<T>(in->mass_function, ml, mu, T(in->m_solar), mm);
+		mean_mass = mm[0];
+		mean_mass2 = mm[1];
 		d->mean_mass2_ln_mass = mm[2];
 	}
 	else
 	{
-		double sp[6];
-		star_params(file->n, file->rectangular, file->corner[0], file->corner[1], file->theta, file->stars, sp);
+		T sp[6];
+		star_params_t<T>(file->n, file->rectangular, T(file->corner[0]), T(file->corner[1]), T(file->theta), file->stars,
+			sp);
 		rect = file->rectangular;
-		th = file->theta;
+		th = T(file->theta);
 		ks = sp[0];
 		d->m_lower = sp[1];
 		d->m_upper = sp[2];
-		d->mean_mass = sp[3];
-		d->mean_mass2 = sp[4];
+		mean_mass = sp[3];
+		mean_mass2 = sp[4];
 		d->mean_mass2_ln_mass = sp[5];
 	}
+	d->mean_mass = mean_mass;
+	d->mean_mass2 = mean_mass2;
 	d->kappa_star = ks;
 	d->theta_star = th;
 	d->rectangular = rect;
 	d->covered = 1;
-	d->mu_ave = 1 / ((1 - kt) * (1 - kt) - g * g);
-	d->num_rays_x = 1.0 * in->num_rays_y * in->num_pixels_y[0] * in->num_pixels_y[1] / (2 * in->half_length_y[0] * 2 * in->half_length_y[1]);
-	double h1 = in->half_length_y[0], h2 = in->half_length_y[1];
-	int n1 = in->num_pixels_y[0], n2 = in->num_pixels_y[1];
-	double rhs1 = std::sqrt(h1 / h2 * n2 / n1);
-	double rhs2 = std::sqrt(h2 / h1 * n1 / n2);
-	double s = 2 * std::sqrt(d->num_rays_x);
+	d->mu_ave = T(1 / ((1 - kt) * (1 - kt) - g * g));
+	const T h1 = T(in->half_length_y[0]), h2 = T(in->half_length_y[1]);
+	const int n1 = in->num_pixels_y[0], n2 = in->num_pixels_y[1];
+	const T nrx = T(1.0 * in->num_rays_y * n1 * n2 / (2 * h1 * 2 * h2));
+	d->num_rays_x = nrx;
+	T rhs1 = std::sqrt(h1 / h2 * n2 / n1);
+	T rhs2 = std::sqrt(h2 / h1 * n1 / n2);
+	const T s = 2 * std::sqrt(nrx);
 	rhs1 /= s;
 	rhs2 /= s;
 	d->ray_half_sep[0] = rhs1;
 	d->ray_half_sep[1] = rhs2;
-	double ext = th * std::sqrt(ks * d->mean_mass2 / (d->mean_mass * in->light_loss));
-	double hx1 = (h1 + ext) / std::abs(1 - kt + g);
-	double hx2 = (h2 + ext) / std::abs(1 - kt - g);
-	int nrt1 = int(hx1 / (2 * rhs1)) + 1;
-	int nrt2 = int(hx2 / (2 * rhs2)) + 1;
-	d->half_length_x[0] = 2 * rhs1 * nrt1;
-	d->half_length_x[1] = 2 * rhs2 * nrt2;
+	const T ext = th * std::sqrt(ks * mean_mass2 / (mean_mass * ll));
+	const T hx1 = (h1 + ext) / std::abs(1 - kt + g);
+	const T hx2 = (h2 + ext) / std::abs(1 - kt - g);
+	const int nrt1 = int(hx1 / (2 * rhs1)) + 1;
+	const int nrt2 = int(hx2 / (2 * rhs2)) + 1;
+	const T hl1 = 2 * rhs1 * nrt1, hl2 = 2 * rhs2 * nrt2;
+	d->half_length_x[0] = hl1;
+	d->half_length_x[1] = hl2;
 	d->num_ray_threads[0] = 2 * nrt1;
 	d->num_ray_threads[1] = 2 * nrt2;
-	d->center_x[0] = in->center_y[0] / (1 - kt + g);
-	d->center_x[1] = in->center_y[1] / (1 - kt - g);
-	double c1 = std::abs(d->center_x[0]) + d->half_length_x[0];
-	double c2 = std::abs(d->center_x[1]) + d->half_length_x[1];
-	double ss = in->safety_scale;
+	const T cx1 = T(in->center_y[0]) / (1 - kt + g);
+	const T cx2 = T(in->center_y[1]) / (1 - kt - g);
+	d->center_x[0] = cx1;
+	d->center_x[1] = cx2;
+	const T c1 = std::abs(cx1) + hl1;
+	const T c2 = std::abs(cx2) + hl2;
+	T k1, k2;   // the field's corner
 	if (!file)
 	{
 		if (rect)
 		{
-			d->num_stars = int(std::ceil(ss * 2 * c1 * ss * 2 * c2 * ks / (PI * th * th * d->mean_mass)));
-			double a = std::sqrt(c1 / c2), b = std::sqrt(c2 / c1);
-			double f = std::sqrt(PI * th * th * d->num_stars * d->mean_mass / (4 * ks));
-			d->corner[0] = a * f;
-			d->corner[1] = b * f;
+			d->num_stars = int(std::ceil(ss * 2 * c1 * ss * 2 * c2 * ks / (pi * th * th * mean_mass)));
+			const T a = std::sqrt(c1 / c2), b = std::sqrt(c2 / c1);
+			const T f = std::sqrt(pi * th * th * d->num_stars * mean_mass / (4 * ks));
+			k1 = a * f;
+			k2 = b * f;
 		}
 		else
 		{
-			double ca = std::hypot(c1, c2);
-			d->num_stars = int(std::ceil(ss * ca * ss * ca * ks / (th * th * d->mean_mass)));
-			double f = std::sqrt(th * th * d->num_stars * d->mean_mass / ks);
-			d->corner[0] = c1 / ca * f;
-			d->corner[1] = c2 / ca * f;
+			const T ca = std::hypot(c1, c2);
+			d->num_stars = int(std::ceil(ss * ca * ss * ca * ks / (th * th * mean_mass)));
+			const T f = std::sqrt(th * th * d->num_stars * mean_mass / ks);
+			k1 = c1 / ca * f;
+			k2 = c2 / ca * f;
 		}
 	}
 	else
 	{
 		d->num_stars = file->n;
-		d->corner[0] = file->corner[0];
-		d->corner[1] = file->corner[1];
-		const double ca = std::hypot(c1, c2);
+		k1 = T(file->corner[0]);
+		k2 = T(file->corner[1]);
+		const T ca = std::hypot(c1, c2);
 		if (!rect)
 		{
-			const double r = std::hypot(file->corner[0], file->corner[1]);
-			d->corner[0] = c1 / ca * r;
-			d->corner[1] = c2 / ca * r;
+			const T r = std::hypot(k1, k2);
+			k1 = c1 / ca * r;
+			k2 = c2 / ca * r;
 		}
-		if ((rect && (d->corner[0] < ss * c1 || d->corner[1] < ss * c2)) ||
-			(!rect && std::hypot(d->corner[0], d->corner[1]) < ss * ca))
+		if ((rect && (k1 < ss * c1 || k2 < ss * c2)) || (!rect && std::hypot(k1, k2) < ss * ca))
 			d->covered = 0;
 	}
-	d->alpha_error = std::min(h1 / (10 * n1), h2 / (10 * n2));
+	d->corner[0] = k1;
+	d->corner[1] = k2;
+	const T ae = std::min(h1 / (10 * n1), h2 / (10 * n2));
+	d->alpha_error = ae;
 	int ts = 1;
-	double cabs_ = std::hypot(d->corner[0], d->corner[1]);
-	double carg_ = std::atan2(d->corner[1], d->corner[0]);
-	while ((ks / PI * 4 / (ts + 1) * cabs_ * (ss + 1) / (ss - 1) * std::pow(1 / ss, ts + 1) > d->alpha_error) && ts <= 101)
+	const T cabs_ = std::hypot(k1, k2);
+	const T carg_ = std::atan2(k2, k1);
+	while ((ks * inv_pi * 4 / (ts + 1) * cabs_ * (ss + 1) / (ss - 1) * std::pow(1 / ss, ts + 1) > ae) && ts <= 101)
 		ts += 2;
-	while ((std::fmod(carg_ * (ts - 1), PI) < 0.1 * PI || std::fmod(carg_ * (ts - 1), PI) > 0.9 * PI) && ts <= 101)
+	while ((std::fmod(carg_ * (ts - 1), pi) < 0.1 * pi || std::fmod(carg_ * (ts - 1), pi) > 0.9 * pi) && ts <= 101)
 		ts += 2;
 	d->taylor_smooth = ts;
 }
 
-int orc_derived_params(const OrcInput* in, OrcDerived* d)
+extern "C" {
+
+/* single = 1: evaluated in float, as the float library derives */
+int orc_derived_params(const OrcInput* in, int single, OrcDerived* d)
 {
-	derive(in, nullptr, d);
+	if (single)
+		derive<float>(in, nullptr, d);
+	else
+		derive<double>(in, nullptr, d);
 	return 0;
 }
 
 /* the star-file branch: n stars (AoS x1, x2, m) with the file's field shape,
    corner and theta; returns 0, or -1 when the field does not cover the
    shooting region (the reference's run then fails, ipm.cuh:521-529) */
-int orc_derived_params_file(const OrcInput* in, int n, int rectangular, double c_re, double c_im, double theta,
-	const double* stars, OrcDerived* d)
+int orc_derived_params_file(const OrcInput* in, int single, int n, int rectangular, double c_re, double c_im,
+	double theta, const double* stars, OrcDerived* d)
 {
 	OrcStarFile f{n, rectangular, {c_re, c_im}, theta, stars};
-	derive(in, &f, d);
+	if (single)
+		derive<float>(in, &f, d);
+	else
+		derive<double>(in, &f, d);
 	return d->covered ? 0 : -1;
 }
 
 /* the corner after the field is realised (ipm.cuh:717-731): the derived
    corner's shape with the size set by the realised mean mass; the stars do
    not move. out = [corner_re, corner_im, root_half_length (ipm.cuh:746-754)] */
-void orc_rescaled_corner(int rectangular, double c_re, double c_im, double theta, int n, double mean_mass_actual,
-	double kappa_star, double* out)
+}  // extern "C"
+
+template <typename T>
+static void rescaled_corner(int rectangular, T c_re, T c_im, T theta, int n, T mean_mass_actual, T kappa_star, double* out)
 {
-	double r, i;
+	T r, i;
 	if (rectangular)
 	{
-		const double f = std::sqrt(PI * theta * theta * n * mean_mass_actual / (4 * kappa_star));
+		const T f = std::sqrt(T(PI) * theta * theta * n * mean_mass_actual / (4 * kappa_star));
 		r = std::sqrt(c_re / c_im) * f;
 		i = std::sqrt(c_im / c_re) * f;
-		out[2] = 1.1 * (r > i ? r : i);
+		out[2] = T((r > i ? r : i) * 1.1);
 	}
 	else
 	{
-		const double a = std::hypot(c_re, c_im);
-		const double f = std::sqrt(theta * theta * n * mean_mass_actual / kappa_star);
+		const T a = std::hypot(c_re, c_im);
+		const T f = std::sqrt(theta * theta * n * mean_mass_actual / kappa_star);
 		r = c_re / a * f;
 		i = c_im / a * f;
-		out[2] = 1.1 * std::hypot(r, i);
+		out[2] = T(std::hypot(r, i) * 1.1);
 	}
 	out[0] = r;
 	out[1] = i;
+}
+
+extern "C" {
+
+/* the tree's root half-length for a field corner (ipm.cuh:746-754): the larger
+   side of a rectangle or the radius of a disk, times 1.1, rounded to T */
+double orc_root_half_length(int rectangular, int single, double c_re, double c_im)
+{
+	if (single)
+	{
+		const float r = float(c_re), i = float(c_im);
+		return float((rectangular ? (r > i ? r : i) : std::hypot(r, i)) * 1.1);
+	}
+	return (rectangular ? (c_re > c_im ? c_re : c_im) : std::hypot(c_re, c_im)) * 1.1;
+}
+
+void orc_rescaled_corner(int rectangular, int single, double c_re, double c_im, double theta, int n,
+	double mean_mass_actual, double kappa_star, double* out)
+{
+	if (single)
+		rescaled_corner<float>(rectangular, float(c_re), float(c_im), float(theta), n, float(mean_mass_actual),
+			float(kappa_star), out);
+	else
+		rescaled_corner<double>(rectangular, c_re, c_im, theta, n, mean_mass_actual, kappa_star, out);
 }
 
 // star.cuh:64-107 with the Philox stream above; out = AoS [x1, x2, m] * n

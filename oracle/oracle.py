@@ -75,9 +75,11 @@ def lib():
         L.orc_mass_moments.argtypes = [i, d, d, d, P(d)]
         L.orc_mass_sample.argtypes = [i, d, d, d, d]
         L.orc_mass_sample.restype = d
-        L.orc_derived_params.argtypes = [P(Input), P(Derived)]
-        L.orc_derived_params_file.argtypes = [P(Input), i, i, d, d, d, P(d), P(Derived)]
-        L.orc_rescaled_corner.argtypes = [i, d, d, d, i, d, d, P(d)]
+        L.orc_derived_params.argtypes = [P(Input), i, P(Derived)]
+        L.orc_derived_params_file.argtypes = [P(Input), i, i, i, d, d, d, P(d), P(Derived)]
+        L.orc_rescaled_corner.argtypes = [i, i, d, d, d, i, d, d, P(d)]
+        L.orc_root_half_length.argtypes = [i, i, d, d]
+        L.orc_root_half_length.restype = d
         L.orc_generate_stars.argtypes = [i, i, d, d, i, d, d, d, ctypes.c_uint32, P(d)]
         L.orc_star_params.argtypes = [i, i, d, d, d, P(d), P(d)]
         L.orc_tree_build.argtypes = [P(d), i, d]
@@ -133,11 +135,14 @@ def mass_sample(mf, p, m_lower, m_upper, m_solar=1.0):
 
 def derived_params(kappa_tot, shear, kappa_star, theta_star=1.0, mass_function="equal", m_solar=1.0,
                    m_lower=0.01, m_upper=50.0, light_loss=0.001, rectangular=0, approx=1, safety_scale=1.37,
-                   center_y=(0.0, 0.0), half_length_y=(5.0, 5.0), num_pixels_y=(1000, 1000), num_rays_y=1):
+                   center_y=(0.0, 0.0), half_length_y=(5.0, 5.0), num_pixels_y=(1000, 1000), num_rays_y=1,
+                   single=False):
+    """Derived geometry (ipm.cuh:370-565); single=True evaluates it in float,
+    as the reference's float build (IPM<float>) does."""
     inp = _input(kappa_tot, shear, kappa_star, theta_star, mass_function, m_solar, m_lower, m_upper, light_loss,
                  rectangular, approx, safety_scale, center_y, half_length_y, num_pixels_y, num_rays_y)
     out = Derived()
-    lib().orc_derived_params(ctypes.byref(inp), ctypes.byref(out))
+    lib().orc_derived_params(ctypes.byref(inp), int(single), ctypes.byref(out))
     return out
 
 
@@ -150,25 +155,30 @@ def _input(kappa_tot, shear, kappa_star, theta_star, mass_function, m_solar, m_l
 
 def derived_params_file(stars, rectangular, corner, theta, kappa_tot, shear, light_loss=0.001, approx=1,
                         safety_scale=1.37, center_y=(0.0, 0.0), half_length_y=(5.0, 5.0), num_pixels_y=(1000, 1000),
-                        num_rays_y=1):
+                        num_rays_y=1, single=False):
     """Derived geometry when the stars come from a file (ipm.cuh:405-432, 507-529):
     the Derived record and whether the file's field covers the shooting region."""
     s = np.ascontiguousarray(stars, dtype=np.float64)
     inp = _input(kappa_tot, shear, 0.0, theta, "equal", 1.0, 1.0, 1.0, light_loss, rectangular, approx,
                  safety_scale, center_y, half_length_y, num_pixels_y, num_rays_y)
     out = Derived()
-    rc = lib().orc_derived_params_file(ctypes.byref(inp), len(s), int(rectangular), corner[0], corner[1], theta,
-                                       _ptr(s), ctypes.byref(out))
+    rc = lib().orc_derived_params_file(ctypes.byref(inp), int(single), len(s), int(rectangular), corner[0], corner[1],
+                                       theta, _ptr(s), ctypes.byref(out))
     return out, rc == 0
 
 
-def rescaled_corner(rectangular, corner, theta, num_stars, mean_mass_actual, kappa_star):
+def rescaled_corner(rectangular, corner, theta, num_stars, mean_mass_actual, kappa_star, single=False):
     """The field corner after the stars are drawn (ipm.cuh:717-731) and the tree's
     root half-length (ipm.cuh:746-754): (corner, root_half_length)."""
     out = np.zeros(3)
-    lib().orc_rescaled_corner(int(rectangular), corner[0], corner[1], theta, int(num_stars), mean_mass_actual,
-                              kappa_star, _ptr(out))
+    lib().orc_rescaled_corner(int(rectangular), int(single), corner[0], corner[1], theta, int(num_stars),
+                              mean_mass_actual, kappa_star, _ptr(out))
     return (out[0], out[1]), out[2]
+
+
+def root_half_length(rectangular, corner, single=False):
+    """The tree's root half-length for a field corner (ipm.cuh:746-754)."""
+    return lib().orc_root_half_length(int(rectangular), int(single), corner[0], corner[1])
 
 
 def generate_stars(n, rectangular, corner, mass_function, m_lower, m_upper, m_solar, seed):

@@ -75,8 +75,36 @@ def test_ipm_histogram_files_are_exact(tmp_path, stars, is_double):
         log = read_hist(prefix + f"ipm_log_mags_numpixels{suffix}.txt")
         np.testing.assert_array_equal(log, expected_hist(llround(np.log10(pos) * 1000.0)), err_msg=f"log{suffix}")
         say(f"[hist{suffix}] {len(lin)} linear bins, {len(log)} log bins, {len(v) - len(pos)} zero pixels")
-    # every pixel is a minimum or a saddle sum, and the saddle map has zero pixels to leave out
-    assert (m.magnifications_saddles == 0).any()
+    m.close()
+
+
+@pytest.mark.parametrize("is_double", [True, False])
+def test_log_histograms_leave_out_zero_pixels(tmp_path, is_double):
+    """One negligible star far outside the field: every source position has a
+    single image, a minimum, so the saddle map is zero everywhere. Its linear
+    histogram is the one bin "0 n1*n2"; its log10 histogram file holds no bin
+    (a zero pixel has no log10), and the total and minima files are those of
+    the returned maps."""
+    from irs_amd.microlensing.IPM.ipm import IPM
+    f = tmp_path / "one_star.bin"
+    with open(f, "wb") as fh:
+        np.array([1, 0], dtype=np.int32).tofile(fh)
+        np.array([1.0e4, 0.0, 1.0], dtype=np.float64).tofile(fh)
+        np.array([1.0e4, 0.0, 1.0e-12], dtype=np.float64).tofile(fh)
+    prefix = str(tmp_path) + "/"
+    say(f"[gpu] one-star histograms, is_double={is_double}")
+    m = IPM(kappa_tot=0.3, shear=0.2, starfile=str(f), half_length_y1=5, half_length_y2=5, num_pixels_y1=40,
+            num_pixels_y2=30, num_rays_y=9, expansion_order=3, is_double=is_double, write_stars=False,
+            write_maps=False, write_parities=True, write_histograms=True, outfile_prefix=prefix)
+    m.run()
+    m.save()
+    assert not m.magnifications_saddles.any() and (m.magnifications_minima > 0).all()
+    np.testing.assert_array_equal(read_hist(prefix + "ipm_mags_numpixels_saddles.txt"), [[0, 40 * 30]])
+    assert os.path.getsize(prefix + "ipm_log_mags_numpixels_saddles.txt") == 0
+    for suffix, mp in (("", m.magnifications), ("_minima", m.magnifications_minima)):
+        v = np.asarray(mp, dtype=np.float64).ravel()
+        np.testing.assert_array_equal(read_hist(prefix + f"ipm_log_mags_numpixels{suffix}.txt"),
+                                      expected_hist(llround(np.log10(v) * 1000.0)), err_msg=suffix)
     m.close()
 
 

@@ -72,6 +72,84 @@ def test_ks_distance_and_features():
     assert np.allclose(f[0], [0.5, 0.2, 0.2, -2.3, 0.08, 1.5])
 
 
+def test_oracle_adam_closed_forms():
+    """The FP64 Adam of the oracle against what the published algorithm fixes:
+    a dropped bias correction fails the constant-gradient case, a sign slip or
+    a transposed moment fails the first step, decay on the biases fails the
+    zero-gradient case."""
+    _, n = mlp.layout()
+    mask = mlp.decay_mask()
+    offs, _ = mlp.layout()
+    assert mask.sum() == sum(b - w for w, b in offs) and mask[offs[0][1]] == 0 and mask[offs[3][0]] == 1
+    rng = np.random.default_rng(8)
+    w0 = rng.normal(size=n)
+    g = rng.normal(size=n) * 10.0 ** rng.uniform(-3, 1, n)
+    lr, eps = 1e-3, 1e-8
+    # first step from zero moments: every weight moves by lr against sign(g) (to eps)
+    w1, m, v = mlp.adam_step(w0, g, np.zeros(n), np.zeros(n), 1, lr, 0.0)
+    np.testing.assert_allclose(w1 - w0, -lr * np.sign(g) * np.abs(g) / (np.abs(g) + eps), rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(m, 0.1 * g, rtol=1e-15)
+    np.testing.assert_allclose(v, 0.001 * g * g, rtol=1e-14)
+    # a constant gradient: the corrected moments equal g and g^2 exactly, so every step is the same
+    w, m, v = w0.copy(), np.zeros(n), np.zeros(n)
+    for t in range(1, 8):
+        w_next, m, v = mlp.adam_step(w, g, m, v, t, lr, 0.0)
+        np.testing.assert_allclose(w_next - w, -lr * g / (np.abs(g) + eps), rtol=1e-9, atol=1e-16)
+        w = w_next
+    # zero gradient with decay: weights shrink geometrically, biases stay
+    w, m, v = w0.copy(), np.zeros(n), np.zeros(n)
+    for t in range(1, 4):
+        w, m, v = mlp.adam_step(w, np.zeros(n), m, v, t, 0.01, 0.1)
+    np.testing.assert_allclose(w, w0 * np.where(mask > 0, (1 - 0.01 * 0.1) ** 3, 1.0), rtol=1e-15)
+    # hand-evaluated second step for one entry: g1 = 2, g2 = -1, lr = 0.1, no decay
+    wa, ma, va = mlp.adam_step(np.full(n, 0.5), np.full(n, 2.0), np.zeros(n), np.zeros(n), 1, 0.1, 0.0)
+    wb, _, _ = mlp.adam_step(wa, np.full(n, -1.0), ma, va, 2, 0.1, 0.0)
+    m2 = 0.9 * 0.2 + 0.1 * -1.0
+    v2 = 0.999 * 0.004 + 0.001 * 1.0
+    want = 0.5 - 0.1 * 2 / (2 + 1e-8) - 0.1 * (m2 / 0.19) / (np.sqrt(v2 / (1 - 0.999 ** 2)) + 1e-8)
+    assert abs(wb[0] - want) < 1e-15
+
+
+@pytest.mark.gpu
+def test_gpu_adam_steps_match_oracle():
+    """Three training steps on the GPU (k_adam over the FP32 master weights)
+    against the oracle's FP64 Adam fed with the GPU's own gradients of the same
+    batch, weight decay on: the parameters after every step agree to FP32
+    rounding of the update, and the BF16 copies the next forward pass uses are
+    refreshed (the step-3 gradients are those of the step-2 weights)."""
+    from irs_amd.microlensing.Surrogate.surrogate import F, lib
+    import ctypes
+    h = lib.surrogate_create(64, 5)
+    n = lib.surrogate_num_params(h)
+    p = random_params(9).astype(np.float32)
+    assert lib.surrogate_set_params(h, p.ctypes.data_as(F))
+    x, t = random_batch(32, 10)
+    xf = np.ascontiguousarray(x, dtype=np.float32)
+    tf = np.ascontiguousarray(t, dtype=np.float32)
+    rows = np.arange(32, dtype=np.int32)
+    lr, wd = 3e-3, 0.05
+    w = p.astype(np.float64)
+    m = np.zeros(n)
+    v = np.zeros(n)
+    for step in range(1, 4):
+        g = np.zeros(n, dtype=np.float32)
+        # the gradients of the current weights (this also makes x, t the resident training set)
+        lib.surrogate_gradients(h, xf.ctypes.data_as(F), tf.ctypes.data_as(F), 32, g.ctypes.data_as(F))
+        loss = lib.surrogate_train_step(h, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 32, lr, wd)
+        assert loss > 0
+        got = np.zeros(n, dtype=np.float32)
+        lib.surrogate_get_params(h, got.ctypes.data_as(F))
+        w, m, v = mlp.adam_step(w, g.astype(np.float64), m, v, step, lr, wd)
+        err = np.abs(got - w)
+        tol = 2e-7 * np.abs(w) + 2e-5 * lr
+        print(f"[gpu] Adam step {step}: max |w_gpu - w_oracle| {err.max():.2e}, worst / tolerance "
+              f"{(err / tol).max():.3f}", flush=True)
+        assert np.all(err <= tol)
+        # continue the oracle from the GPU's FP32 weights, as the GPU does
+        w = got.astype(np.float64)
+    lib.surrogate_free(h)
+
+
 @pytest.mark.gpu
 def test_gpu_gradients_match_oracle():
     from irs_amd.microlensing.Surrogate.surrogate import F, lib
