@@ -333,6 +333,44 @@ def ncc_bench(num_phi=100, seed=3, runs=5):
     return out
 
 
+def direct_sum_check(m, mir, npoints, seed=77):
+    """Deflections at seeded lattice corners, FP64 and mixed precision, against
+    the host's FP64 direct sum over every star (alpha_star.cuh:18-41 without the
+    tree); the reference sizes its FMM for |error| <= alpha_error
+    (ipm.cuh:532-533). Near a star the mixed path's FP32 near field adds a
+    rounding term of the size of the FP32 spacing of the leaf-relative offsets
+    times sum m / |z - z_i|^2; the worst point is reported with its nearest
+    star and that sum."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    n1, n2 = m.num_ray_threads
+    g = mir.grid
+    I = rng.integers(0, n1 + 1, npoints)
+    J = rng.integers(0, n2 + 1, npoints)
+    z = (g.center_x[0] - g.half_length_x[0] + 2 * g.ray_half_sep[0] * I) + \
+        1j * (g.center_x[1] - g.half_length_x[1] + 2 * g.ray_half_sep[1] * J)
+    t0 = time.time()
+    wd = O.deflect_direct(mir.stars, mir.lens, z)
+    t_direct = time.time() - t0
+    alpha = float(m.alpha_error)
+    out = {"points": int(npoints), "alpha_error": alpha, "oracle_s": t_direct,
+           "oracle_threads": O.lib().orc_num_threads(), "oracle": "FP64 direct sum over every star"}
+    keep = m.precision
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        m.precision = prec
+        dw = np.abs(m.deflect(z) - wd)
+        k = int(np.argmax(dw))
+        d2 = (mir.stars[:, 0] - z[k].real) ** 2 + (mir.stars[:, 1] - z[k].imag) ** 2
+        near = np.argsort(d2)[:64]
+        out[name] = {"median": float(np.median(dw)), "p99": float(np.percentile(dw, 99)), "max": float(dw.max()),
+                     "above_alpha_error": int(np.sum(dw > alpha)),
+                     "worst_nearest_star_distance": float(np.sqrt(d2[near[0]])),
+                     "worst_sum_m_over_r2": float(np.sum(mir.stars[near, 2] / d2[near]))}
+    m.precision = keep
+    log(f"[bench] direct-sum check {out}")
+    return out
+
+
 def oracle_window(m, mir, maps, x0, y0, size=512):
     """A size x size window of the map re-shot by the FP64 oracle on the host:
     the GPU lists every lattice cell that can reach the window (mixed-precision
@@ -367,6 +405,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--ref-rows", type=int, default=4096, help="rows for the reference-faithful FP64 kernel")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--direct-points", type=int, default=20000,
+                    help="lattice corners checked against the host's direct sum over every star")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-irs", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -455,13 +495,21 @@ def main():
         err_win = float(np.median(e[y0:y0 + 512, x0:x0 + 512]))
         out["fp64_max_rel_err"] = float(np.max(e))
         log(f"[bench] mixed vs fp64 median rel err {err_median:.3e} (512^2 window {err_win:.3e}), max {np.max(e):.3e}")
-        # reference-faithful FP64 kernel on a block of rows in the middle of the lattice
+        # reference-faithful FP64 kernel on seeded blocks of rows spread over the whole lattice
         n2 = m.num_ray_threads[1]
-        r0 = n2 // 2 - args.ref_rows // 2
+        block = 64
+        starts = np.sort(np.random.default_rng(5).choice(n2 // block, size=max(1, args.ref_rows // block),
+                                                        replace=False)) * block
         m.kernel = 1
-        m.shoot(r0, r0 + args.ref_rows)
-        ref_rate = m.cells_shot / m.t_shoot_cells
-        log(f"[bench] reference fp64 kernel: {m.cells_shot} cells in {m.t_shoot_cells:.3f} s")
+        ref_cells, ref_s = 0, 0.0
+        for r0 in starts:
+            m.shoot(int(r0), int(r0) + block)
+            ref_cells += m.cells_shot
+            ref_s += m.t_shoot_cells
+        ref_rate = ref_cells / ref_s
+        out["reference_fp64_sample"] = (f"{len(starts)} seeded blocks of {block} lattice rows spread over all "
+                                        f"{n2} rows, {ref_cells} cells")
+        log(f"[bench] reference fp64 kernel: {ref_cells} cells in {ref_s:.3f} s ({len(starts)} blocks)")
         m.kernel = 0
         m.precision = 1
 
@@ -509,6 +557,8 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU oracle: tree and FMM")
         mir, t_setup = oracle_mirror(m, P_ORDER)
+        log("[bench] CPU oracle: direct sums at sampled lattice corners")
+        result["direct_sum_check"] = direct_sum_check(m, mir, args.direct_points)
         log("[bench] CPU oracle: 512^2 window")
         result["cpu_oracle_window"] = oracle_window(m, mir, maps, x0, y0)
         log("[bench] CPU oracle baseline")
@@ -525,8 +575,16 @@ def main():
         r.build()
         r.shoot()
         r.shoot()
+        irs_rate = r.cells_shot / r.t_shoot_rays
+        fpr = flop_per_cell(r.mean_near_stars, P_ORDER)
+        irs_tf = irs_rate * fpr / 1e12
         result["irs"] = {"rays": r.cells_shot, "num_stars": r.num_stars, "shoot_s": r.t_shoot_rays,
-                         "rays_per_s": r.cells_shot / r.t_shoot_rays}
+                         "rays_per_s": irs_rate, "mean_near_stars": r.mean_near_stars,
+                         "roofline": {"kernel": "k_irs_fast", "bound": "valu", "achieved": irs_tf,
+                                      "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s", "frac": irs_tf / PEAK_VALU_TFLOPS,
+                                      "flop_per_ray": fpr,
+                                      # int32 atomics that leave the LDS window, 8 B each, at most one per ray
+                                      "atomic_bytes_per_ray_max": 8}}
         log(f"[bench] IRS: {r.cells_shot} rays in {r.t_shoot_rays:.3f} s")
         r.close()
     print(json.dumps(result), flush=True)

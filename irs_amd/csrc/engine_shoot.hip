@@ -667,7 +667,7 @@ __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long l
 	const int wave = threadIdx.x >> 6;
 	const long long I = tile_x * (63 * WAVES) + wave * 63 + lane;
 	const long long Ic = I < a.N1 ? I : a.N1;
-	const double zx = a.x0 + a.stepx * double(Ic);
+	const double zx = lattice_at(a.x0, a.stepx, Ic);
 	const bool cell_ok = lane < 63 && I < a.N1;
 	LeafCache cache;
 	using MapT = typename std::conditional<PREC == PREC_MIXED, float, double>::type;
@@ -675,7 +675,7 @@ __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long l
 	int qn = 0;
 
 	auto corner = [&](long long J, double& px, double& py) {
-		double zy = a.y0 + a.stepy * double(J);
+		double zy = lattice_at(a.y0, a.stepy, J);
 		Deflection d = deflect<PREC, false>(a, zx, zy, cache);
 		// ipm_functions.cuh:159-165 and array_functions.cuh:70-75
 		px = ((d.wx - a.cyx) + a.h1) * a.s1;
@@ -754,11 +754,11 @@ __global__ __launch_bounds__(64 * IPM_WAVES) void k_window_cells(ShootArgs a, lo
 	const long long Ibase = (long long)blockIdx.x * IPM_TILE_W + wave * 63;
 	const long long I = Ibase + lane;
 	const long long Ic = I < a.N1 ? I : a.N1;
-	const double zx = a.x0 + a.stepx * double(Ic);
+	const double zx = lattice_at(a.x0, a.stepx, Ic);
 	const bool cell_ok = lane < 63 && I < a.N1;
 	LeafCache cache;
 	auto corner = [&](long long J, double& px, double& py) {
-		double zy = a.y0 + a.stepy * double(J);
+		double zy = lattice_at(a.y0, a.stepy, J);
 		Deflection d = deflect<PREC_MIXED, false>(a, zx, zy, cache);
 		px = ((d.wx - a.cyx) + a.h1) * a.s1;
 		py = double(a.n2) - ((d.wy - a.cyy) + a.h2) * a.s2;
@@ -1258,7 +1258,7 @@ __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long
 	if (J0 >= row1) return;
 	const long long J1 = min(J0 + tile_h, row1);
 	const bool ok = i < a.N1;
-	const double zx = a.x0 + a.stepx * double(ok ? i : a.N1 - 1);
+	const double zx = lattice_at(a.x0, a.stepx, ok ? i : a.N1 - 1);
 	LeafCache cache;
 	unsigned long long got = 0;
 	for (int k = threadIdx.x; k < IRS_WX * IRS_WY; k += IRS_THREADS) win_counts[k] = 0;
@@ -1275,7 +1275,7 @@ __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long
 	IrsWindow win{win_counts, 0, 0};
 	for (long long j = J0; j < J1; j++)
 	{
-		double zy = a.y0 + a.stepy * double(j);
+		double zy = lattice_at(a.y0, a.stepy, j);
 		Deflection d = deflect<PREC, PARITY>(a, zx, zy, cache);
 		int px = 0, py = 0;
 		const bool hit = ok && !isnan(d.wx) && !isnan(d.wy) && irs_pixel(a, d.wx, d.wy, px, py);
@@ -1356,8 +1356,8 @@ __global__ void k_irs_positions(ShootArgs a, long long row, double* w, long long
 {
 	long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	bool ok = i < a.N1;
-	double zx = a.x0 + a.stepx * double(ok ? i : a.N1 - 1);
-	double zy = a.y0 + a.stepy * double(row);
+	double zx = lattice_at(a.x0, a.stepx, ok ? i : a.N1 - 1);
+	double zy = lattice_at(a.y0, a.stepy, row);
 	LeafCache cache;
 	Deflection d = deflect<PREC, false>(a, zx, zy, cache);
 	if (!ok) return;

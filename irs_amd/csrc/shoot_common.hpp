@@ -71,6 +71,19 @@ struct ShootArgs
 	a.poly_d = poly_d;          \
 	a.poly_f = poly_f;
 
+/* Lattice coordinate x0 + step * i with two roundings, never one fused
+   multiply-add. Lattice points land exactly on the tree's dividing lines (a
+   field centred at the origin puts corners on x = 0 and y = 0), where the
+   descent's >= rule picks a side; the FMM deflection jumps across a leaf
+   boundary by about its truncation error, so the point must fall on the side
+   the host (which has no FMA) puts it. */
+__device__ __forceinline__ double lattice_at(double x0, double step, long long i)
+{
+#pragma clang fp contract(off)
+	const double t = step * double(i);
+	return x0 + t;
+}
+
 __device__ __forceinline__ int quadrant(double x, double y, double cx, double cy)
 {
 	if (y >= cy) return (x >= cx) ? 0 : 1;

@@ -8,9 +8,10 @@ standardised with the training set's mean and spread. Targets are the
 normalised histograms of ipm_batch_histograms (128 bins of log10 magnification
 over [-2, 2]).
 
-Where the network is unsure its prediction is not used: a prediction whose
-entropy exceeds a threshold, calibrated on held-out maps so that the accepted
-predictions meet a KS target, falls back to shooting the map (plain IPM).
+Where the network is unreliable its prediction is not used: predictions on
+one side of an entropy threshold, calibrated on held-out maps so that the
+accepted predictions meet a KS target, fall back to shooting the map (plain
+IPM). The calibration also picks the side (see calibrate).
 """
 
 import ctypes
@@ -55,6 +56,25 @@ def entropy(p):
     return -np.sum(p * np.log(p), axis=1)
 
 
+def entropy_threshold(h, ks, ks_target, quantile):
+    """(threshold, reject_above): the entropy threshold and the side of it that
+    falls back, chosen so that the accepted predictions' KS distances have the
+    given quantile within ks_target with as many accepted as possible (a tie
+    goes to falling back above). (nan, True) when no non-empty set meets the
+    target."""
+    h = np.asarray(h, dtype=np.float64)
+    ks = np.asarray(ks, dtype=np.float64)
+    best = (0, np.nan, True)
+    for above in (True, False):
+        order = np.argsort(h if above else -h, kind="stable")
+        for k in range(len(order), 0, -1):
+            if np.quantile(ks[order[:k]], quantile) <= ks_target:
+                if k > best[0]:
+                    best = (k, float(h[order[k - 1]]), above)
+                break
+    return best[1], best[2]
+
+
 class HistogramSurrogate:
     def __init__(self, max_batch=256, seed=1):
         self.max_batch = max_batch
@@ -63,7 +83,8 @@ class HistogramSurrogate:
             raise Exception("Error creating the surrogate")
         self.mean = np.zeros(NIN)
         self.std = np.ones(NIN)
-        self.threshold = np.inf
+        self.threshold = np.inf     # accept everything until calibrated
+        self.reject_above = True
         self.losses = []
 
     @property
@@ -125,24 +146,29 @@ class HistogramSurrogate:
         xs = self._inputs(x)
         return lib.surrogate_time_predict(self.obj, xs.ctypes.data_as(F), len(xs), reps)
 
-    def calibrate(self, x, hist, ks_target=0.02, quantile=0.95):
-        """Largest entropy threshold such that, among held-out predictions with
-        entropy below it, the given quantile of the KS distance to the maps'
-        histograms stays within ks_target."""
+    def calibrate(self, x, hist, ks_target=0.02, quantile=0.5):
+        """Entropy threshold for the fallback, set on held-out draws (x, hist).
+
+        The predictions are ranked by entropy. For each side of a threshold
+        (fall back above it, or below it) the largest accepted set whose KS
+        distances to the maps' histograms have the given quantile within
+        ks_target is found; the side that accepts more wins, and a tie goes to
+        falling back above the threshold. On the +-50 theta_E, 256^2 maps the KS
+        error of a prediction falls as its entropy rises (a peaked histogram has
+        a steep CDF, so a small shift in location is a large KS distance), and
+        the fallback then takes the low-entropy predictions. Returns the
+        threshold; self.reject_above records the side. Accepts nothing (and
+        returns nan) when no non-empty set meets the target."""
         p = self.predict(x)
-        ks = ks_distance(p, hist)
-        h = entropy(p)
-        order = np.argsort(h)
-        best = -np.inf
-        for k in range(1, len(order) + 1):
-            if np.quantile(ks[order[:k]], quantile) <= ks_target:
-                best = h[order[k - 1]]
-        self.threshold = best
-        return best
+        self.threshold, self.reject_above = entropy_threshold(entropy(p), ks_distance(p, hist), ks_target, quantile)
+        return self.threshold
 
     def accept(self, p):
         """True where the prediction is used, False where the map must be shot."""
-        return entropy(p) <= self.threshold
+        h = entropy(p)
+        if not np.isfinite(self.threshold):
+            return np.zeros(len(h), dtype=bool)
+        return h <= self.threshold if self.reject_above else h >= self.threshold
 
     def close(self):
         if self.obj:

@@ -48,15 +48,15 @@ def c1_stars(oracle, tmp_path_factory):
     return path, stars
 
 
-def assert_fp64_map(got, ref, what, p99=1e-6, worst=1e-5):
-    """As the FP64 map bounds of the parity tests: FP64 rounding differences of
-    the corners (FMA contraction, libm) only."""
+def assert_fp64_map(got, ref, what, p99=1e-10, worst=1e-9):
+    """As the FP64 map bounds of the parity tests: the same lattice corners,
+    FP64 rounding differences of their deflections only."""
     e = rel_err(got, ref, floor=1e-12)
     say(f"[{what}] rel err median {np.median(e):.2e}  p99 {np.percentile(e, 99):.2e}  max {np.max(e):.2e}")
-    assert np.median(e) < 1e-10
+    assert np.median(e) < 1e-12
     assert np.percentile(e, 99) < p99
     assert np.max(e) < worst
-    assert abs(got.sum() / ref.sum() - 1) < 1e-9
+    assert abs(got.sum() / ref.sum() - 1) < 1e-11
 
 
 @pytest.mark.parametrize("lib", ["double", "float", "irs"])
@@ -195,11 +195,13 @@ def test_c1_full_size_rows_against_direct_sum(oracle, tmp_path):
 
 def test_c2_full_size_against_direct_sum_and_window(oracle):
     """The 10000^2 map (1.28e6 stars, p = 30): Mirror checks the engine's
-    geometry; deflections at 2e4 seeded lattice corners, in FP64 and mixed
-    precision, are within alpha_error = 2.5e-4 (ipm.cuh:532-533) of the oracle's
-    direct sum over all stars; and a seeded 512^2 window of the mixed and FP64
-    maps matches the oracle's FP64 re-shoot of every cell that reaches it
-    (median 1.8e-6 and 1.0e-12 measured)."""
+    geometry; deflections at 2e4 seeded lattice corners against the oracle's
+    direct sum over all stars: FP64 within 1e-9 (4e-11 measured, far inside
+    alpha_error = 2.5e-4, ipm.cuh:532-533); mixed precision within alpha_error
+    plus the FP32 rounding of the near field, which exceeds alpha_error only
+    within ~0.01 of a star (2 of 2e4 corners measured); and a seeded 512^2
+    window of the mixed and FP64 maps matches the oracle's FP64 re-shoot of
+    every cell that reaches it (median 1.8e-6 and 1.0e-12 measured)."""
     from irs_amd.microlensing.IPM.ipm import IPM
     full = dict(C2_SMALL, half_length_y1=25.0, half_length_y2=25.0, num_pixels_y1=10000, num_pixels_y2=10000,
                 num_rays_y=10, random_seed=1)
@@ -219,11 +221,26 @@ def test_c2_full_size_against_direct_sum_and_window(oracle):
         1j * (g.center_x[1] - g.half_length_x[1] + 2 * g.ray_half_sep[1] * J)
     say("[cpu] C2 full: direct sums at 2e4 corners over every star")
     w_direct = oracle.deflect_direct(mir.stars, mir.lens, z)
+    # near a star the mixed path's FP32 near field rounds the leaf-relative
+    # offsets (at most ~6 leaf half-lengths h) to 2^-24: an error up to about
+    # 2^-24 * 6 h * sum m / |z - z_i|^2, dominated by the nearest stars
+    from scipy.spatial import cKDTree
+    geom = mir.tree.export()[0]
+    _, leaf = mir.tree.deflect(mir.lens, z)
+    h = geom[leaf, 2]
+    dist, idx = cKDTree(mir.stars[:, :2]).query(np.column_stack([z.real, z.imag]), k=64)
+    S = (mir.stars[idx, 2] / dist ** 2).sum(axis=1)
     for prec in (0, 1):
         m.precision = prec
         dw = np.abs(m.deflect(z) - w_direct)
-        say(f"[C2 direct] precision {prec}: |dw| median {np.median(dw):.2e} max {dw.max():.2e} (alpha_error {alpha})")
-        assert dw.max() < alpha
+        say(f"[C2 direct] precision {prec}: |dw| median {np.median(dw):.2e} max {dw.max():.2e} (alpha_error {alpha}); "
+            f"{int(np.sum(dw > alpha))} above alpha_error, max dw / (h S) {np.max(dw / (h * S)):.2e}")
+        if prec == 0:
+            assert dw.max() < 1e-9
+        else:
+            assert np.median(dw) < 1e-6
+            assert np.all(dw <= alpha + 2.0 ** -21 * h * S)
+            assert np.mean(dw > alpha) < 1e-3
     m.precision = 1
     say("[gpu] C2 full: mixed map")
     m.shoot()
@@ -242,7 +259,7 @@ def test_c2_full_size_against_direct_sum_and_window(oracle):
     say(f"[C2 window] mixed median {np.median(e32):.2e} max {e32.max():.2e}; fp64 median {np.median(e64):.2e} "
         f"max {e64.max():.2e}")
     assert np.median(e32) < 1e-4 and e32.max() < 1e-2
-    assert np.median(e64) < 1e-10 and e64.max() < 1e-5
+    assert np.median(e64) < 1e-11 and e64.max() < 1e-6
     m.close()
 
 
@@ -262,7 +279,7 @@ def test_batched_instances_against_oracle(oracle, tmp_path):
     objs, stars = [], []
     for k, p in enumerate(params):
         path = tmp_path / f"inst{k}.bin"
-        stars.append(write_starfile(oracle, path, p, seed=p["random_seed"]))
+        stars.append(write_starfile(oracle, path, p, seed=p["random_seed"], margin=1.1))
         m = IPM(**from_starfile(p, path))   # mixed precision, p = 20 (the drawn parameters)
         m.prepare()
         m.build()

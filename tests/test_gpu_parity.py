@@ -208,21 +208,21 @@ def test_magnification_matches_oracle(c1_fp64, c1_mirror):
     assert np.all(np.sign(mu_gpu) == np.sign(mu_ref))
 
 
-def assert_fp64_map(got, ref, what):
-    """FP64 map against the FP64 oracle. Both evaluate the same algorithm; they
-    differ only in rounding (the device contracts a*b+c into FMAs, uses its own
-    libm and sums in another order). The map amplifies incoherent last-bit
-    differences of the corners: the oracle compiled with FMA contraction
-    differs from the stock oracle on this C1 map by median 1e-11, p99.9 8e-7,
-    max 1.9e-6. The bounds below sit just outside that envelope."""
+def assert_fp64_map(got, ref, what, worst=1e-9, p99=1e-10, median=1e-12, flux=1e-11):
+    """FP64 map against the FP64 oracle. Both evaluate the same algorithm on
+    bit-identical lattice corners; the deflections differ in rounding (the
+    device fuses multiply-adds in the near field and the Horner sums, and sums
+    in another order), which moves the map by median ~1e-14 and at most ~6e-11
+    on the small cases (1e-8 over the 2.99e8 cells of the full small case).
+    A dropped, doubled or misplaced contribution moves a pixel by 1e-3 or more."""
     assert got.shape == ref.shape
     e = rel_err(got, ref, floor=1e-12)
     say(f"[{what}] rel err median {np.median(e):.2e}  p99 {np.percentile(e, 99):.2e}  "
-        f"p99.9 {np.percentile(e, 99.9):.2e}  max {np.max(e):.2e}")
-    assert np.median(e) < 1e-10
-    assert np.percentile(e, 99) < 1e-6
-    assert np.max(e) < 1e-5
-    assert abs(got.sum() / ref.sum() - 1) < 1e-9   # FMA-contracted oracle: 1.39e-10
+        f"p99.9 {np.percentile(e, 99.9):.2e}  max {np.max(e):.2e}; flux {got.sum() / ref.sum() - 1:.2e}")
+    assert np.median(e) < median
+    assert np.percentile(e, 99) < p99
+    assert np.max(e) < worst
+    assert abs(got.sum() / ref.sum() - 1) < flux
 
 
 def test_ipm_fp64_map_matches_oracle(c1_fp64, c1_mirror):
@@ -230,8 +230,14 @@ def test_ipm_fp64_map_matches_oracle(c1_fp64, c1_mirror):
 
 
 def test_ipm_reference_kernel_matches_oracle(c1_stars, c1_mirror):
+    """The reference-faithful kernel forms each corner as the cell centre
+    plus or minus the half-separation and divides by Smith's formula
+    (ipm_functions.cuh:125-166, complex.cuh:267-280), as the reference does:
+    its corners differ from the lattice points in the last bits, which the
+    map amplifies to p99 8e-8, max 1.2e-6 (measured)."""
     m = run_ipm(from_starfile(C1_SMALL, c1_stars[0]), is_double=True, expansion_order=16, kernel=1)
-    assert_fp64_map(m.magnifications, c1_mirror.ipm_map(), "reference fp64")
+    assert_fp64_map(m.magnifications, c1_mirror.ipm_map(), "reference fp64", worst=1e-5, p99=1e-6, median=1e-10,
+                    flux=1e-9)
     m.close()
 
 
@@ -262,7 +268,7 @@ def test_c1_full_size_maps_match_oracle(oracle, tmp_path):
     assert a.cells_shot == 10930 * 27322
     say("[cpu] C1 full size, oracle map")
     ref = Mirror(oracle, a).ipm_map()
-    assert_fp64_map(a.magnifications, ref, "C1 full fp64")
+    assert_fp64_map(a.magnifications, ref, "C1 full fp64", worst=1e-7)
     say("[gpu] C1 full size, mixed")
     b = run_ipm(from_starfile(params, path), is_double=False, expansion_order=25)
     e = rel_err(b.magnifications, ref, floor=1e-12)
@@ -324,7 +330,7 @@ def test_window_cells_cover_the_window(c1_fp64, c1_mirror):
     # the per-pixel bounds of whole maps (a window this small does not average the flux to 1e-9)
     e = rel_err(c1_fp64.magnifications[y0:y0 + h, x0:x0 + w], win, floor=1e-12)
     say(f"[fp64 window] rel err median {np.median(e):.2e}  max {np.max(e):.2e}")
-    assert np.median(e) < 1e-10 and np.percentile(e, 99) < 1e-6 and np.max(e) < 1e-5
+    assert np.median(e) < 1e-12 and np.percentile(e, 99) < 1e-10 and np.max(e) < 1e-9
     # a too small record buffer reports the full count
     import ctypes
     small = np.zeros((4, 3), dtype=np.int64)

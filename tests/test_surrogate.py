@@ -110,6 +110,27 @@ def test_oracle_adam_closed_forms():
     assert abs(wb[0] - want) < 1e-15
 
 
+def test_entropy_threshold_picks_the_side_that_keeps_more():
+    from irs_amd.microlensing.Surrogate.surrogate import entropy_threshold
+    rng = np.random.default_rng(3)
+    h = rng.uniform(2, 4, 400)
+    # error grows with entropy: fall back above a threshold
+    ks = 0.005 + 0.01 * (h - 2) + rng.uniform(0, 0.002, 400)
+    t, above = entropy_threshold(h, ks, 0.015, 0.9)
+    acc = h <= t
+    assert above and 0 < acc.mean() < 1 and np.quantile(ks[acc], 0.9) <= 0.015
+    # error falls as entropy rises (the +-50 theta_E maps): fall back below it
+    ks2 = 0.005 + 0.01 * (4 - h) + rng.uniform(0, 0.002, 400)
+    t, above = entropy_threshold(h, ks2, 0.015, 0.9)
+    acc = h >= t
+    assert not above and 0 < acc.mean() < 1 and np.quantile(ks2[acc], 0.9) <= 0.015
+    # the larger accepted set wins; nothing meets an impossible target
+    assert np.isnan(entropy_threshold(h, ks, 0.001, 0.5)[0])
+    # a flat error accepts everything, falling back above by the tie rule
+    t, above = entropy_threshold(h, np.full(400, 0.01), 0.02, 0.9)
+    assert above and t == h.max()
+
+
 @pytest.mark.gpu
 def test_gpu_adam_steps_match_oracle():
     """Three training steps on the GPU (k_adam over the FP32 master weights)
@@ -234,4 +255,47 @@ def test_gpu_surrogate_beats_mean_histogram_on_ipm_maps():
           f"{t_pred * 1e6:.2f} us per histogram vs {t_ipm / len(params) * 1e3:.3f} ms per batched map", flush=True)
     assert np.median(ks) < 0.5 * np.median(base)
     assert t_pred < t_ipm / len(params)
+    s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_at_the_surrogate_geometry():
+    """The surrogate's own map geometry (256^2 over +-50 theta_E, 4 rays per
+    pixel, mass function, m_lower and m_upper drawn as well; the full-size
+    configuration is bench.py --surrogate): 3000 training maps, 512 to
+    calibrate the fallback, 512 held out. Measured: KS median 0.029 against the
+    held-out maps, 0.016 between two maps of the same parameters, 0.37 for the
+    mean training histogram; 0.25 us per predicted histogram against 0.42 ms
+    per batched map. The fallback, calibrated to a KS median below that of all
+    calibration predictions, must shoot some maps and leave the accepted
+    predictions closer to their maps than the whole held-out set."""
+    from irs_amd.microlensing.Surrogate import data
+    from irs_amd.microlensing.Surrogate import surrogate as S
+    n_train, n_cal, n_test = 3000, 512, 512
+    params = data.draw_params(n_train + n_cal + n_test, seed=11)
+    print("[gpu] surrogate geometry: shooting the maps", flush=True)
+    hist, t_ipm = data.histograms(params)
+    test = params[n_train + n_cal:]
+    hist2, _ = data.histograms([dict(p, random_seed=p["random_seed"] + 7919) for p in test])
+    x = S.features(params)
+    tr, ca, te = slice(0, n_train), slice(n_train, n_train + n_cal), slice(n_train + n_cal, None)
+    s = S.HistogramSurrogate(max_batch=256, seed=1)
+    s.fit(x[tr], hist[tr], epochs=300)
+    p = s.predict(x[te])
+    ks = S.ks_distance(p, hist[te])
+    floor = S.ks_distance(hist2, hist[te])
+    base = S.ks_distance(np.tile(S.normalise(hist[tr]).mean(axis=0), (n_test, 1)), hist[te])
+    speedup = (t_ipm / len(params)) / s.seconds_per_histogram(x[te])
+    ks_cal = S.ks_distance(s.predict(x[ca]), hist[ca])
+    s.calibrate(x[ca], hist[ca], ks_target=0.8 * float(np.median(ks_cal)), quantile=0.5)
+    acc = s.accept(p)
+    print(f"[gpu] surrogate: KS median {np.median(ks):.4f} (maps of the same parameters {np.median(floor):.4f}, "
+          f"mean histogram {np.median(base):.4f}); {speedup:.0f}x per map; fallback "
+          f"{'above' if s.reject_above else 'below'} entropy {s.threshold:.3f}, accepts {acc.mean():.2f} with KS "
+          f"median {np.median(ks[acc]) if acc.any() else float('nan'):.4f}", flush=True)
+    assert np.median(ks) < 0.045 and np.median(ks) < 3 * np.median(floor)
+    assert np.median(ks) < 0.15 * np.median(base)
+    assert speedup >= 100
+    assert 0 < acc.mean() < 1
+    assert np.median(ks[acc]) < np.median(ks)
     s.close()
