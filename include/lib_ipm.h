@@ -155,6 +155,14 @@ long long get_cells_shot(IPMHandle* self);
 /* lens equation through the built tree at n points z (re, im pairs) into w;
    mu (may be NULL) receives the magnification, FP64 */
 bool ipm_deflect(IPMHandle* self, const double* z, int n, double* w, double* mu);
+/* The same as ipm_deflect for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc); it has no counterpart in the reference. It calls no allocating
+   routine: the points, deflections and magnifications are staged in a fixed
+   array of at most 65536 points (2.6 MB) that each GPU running the library
+   holds until the process ends. n above 65536 is refused (false, message on
+   stderr). Calls that overlap on one GPU share that array and race. */
+bool ipm_deflect_noalloc(IPMHandle* self, const double* z, int n, double* w, double* mu);
 /* for checking a window of a large map against a host FP64 re-shoot of just the
    cells that reach it: the cells whose mixed-precision corners' pixel box meets
    columns [x0, x0 + width) and rows [y0, y0 + height) (row 0 at the top), widened

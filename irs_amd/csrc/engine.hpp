@@ -142,6 +142,11 @@ public:
 
 	/* thin test entry: lens equation at arbitrary points through the tree */
 	void deflect_points(const LensParams& lens, const double* z, int n, Precision prec, double* w, double* mu);
+	/* the same without allocating: the points, deflections and magnifications
+	   are staged in a fixed device array of DEFLECT_NOALLOC_MAX points (one per
+	   device, kept until the process ends); calls that overlap on one device race */
+	static constexpr int DEFLECT_NOALLOC_MAX = 1 << 16;
+	void deflect_points_noalloc(const LensParams& lens, const double* z, int n, Precision prec, double* w, double* mu);
 
 	/* the per-ray source position of IRS rows (test entry for bit-exact binning) */
 	void irs_positions(const LensParams& lens, const GridParams& g, Precision prec, long long row, double* w);
@@ -167,6 +172,7 @@ private:
 
 	hipStream_t stream_ = nullptr;
 	int num_cus_ = 0;   // compute units of the current device
+	int dev_ = 0;       // the device current when the engine was made
 	int tile_rows_ = 0, last_tile_rows_ = 0;
 	hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 

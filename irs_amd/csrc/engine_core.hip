@@ -47,6 +47,7 @@ Engine::Engine()
 	MLENS_HIP_CHECK(hipEventCreate(&ev1_));
 	int dev = 0;
 	MLENS_HIP_CHECK(hipGetDevice(&dev));
+	dev_ = dev;
 	MLENS_HIP_CHECK(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, dev));
 }
 

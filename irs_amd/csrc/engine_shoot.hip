@@ -17,6 +17,8 @@
 #include <chrono>
 #include <iostream>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -1602,6 +1604,76 @@ void Engine::deflect_points(const LensParams& lens, const double* z, int n, Prec
 	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
 	MLENS_HIP_CHECK(hipMemcpy(w, dw.ptr, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
 	if (mu) MLENS_HIP_CHECK(hipMemcpy(mu, dmu.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+}
+
+/*
+Deflection without allocation. One call's points, deflections and
+magnifications are staged in `scratch`, a fixed array in the device's global
+memory (2.6 MB) that every GPU running this library holds until the process
+ends. Host code never uses the array's name as an address: the device address
+is looked up once per device, with that device current, and kept. Calls that
+overlap on one device share the array and race.
+*/
+static __device__ __align__(256) double scratch[5 * Engine::DEFLECT_NOALLOC_MAX];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+
+// the device address of `scratch` on device dev, which must be current
+static double* scratch_on(int dev)
+{
+	static std::mutex mtx;
+	static double* addr[SCRATCH_MAX_DEVICES] = {};
+	if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) throw HipError("device index beyond the scratch table", __FILE__, __LINE__);
+	std::lock_guard<std::mutex> lock(mtx);
+	if (!addr[dev])
+	{
+		void* p = nullptr;
+		MLENS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		addr[dev] = static_cast<double*>(p);
+	}
+	return addr[dev];
+}
+
+void Engine::deflect_points_noalloc(const LensParams& lens, const double* z, int n, Precision prec, double* w, double* mu)
+{
+	if (n < 0 || n > DEFLECT_NOALLOC_MAX)
+		throw HipError("ipm_deflect_noalloc takes 0 to " + std::to_string(DEFLECT_NOALLOC_MAX) + " points per call, not " +
+				std::to_string(n),
+			__FILE__, __LINE__);
+	if (n == 0) return;
+	int cur = 0;
+	MLENS_HIP_CHECK(hipGetDevice(&cur));
+	if (cur != dev_) MLENS_HIP_CHECK(hipSetDevice(dev_));
+	struct Restore
+	{
+		int dev;
+		bool on;
+		~Restore()
+		{
+			if (on) (void)hipSetDevice(dev);
+		}
+	} restore{cur, cur != dev_};
+	double* base = scratch_on(dev_);
+	double* dz = base;
+	double* dw = base + 2 * DEFLECT_NOALLOC_MAX;
+	double* dmu = base + 4 * DEFLECT_NOALLOC_MAX;
+	GridParams g;
+	std::memset(&g, 0, sizeof(g));
+	g.half_length_y[0] = g.half_length_y[1] = 1;
+	g.num_pixels[0] = g.num_pixels[1] = 1;
+	ShootArgs a = ShootArgsBuilder::make(lens, g, false);
+	MLENS_TREE_ARGS(a)
+	MLENS_HIP_CHECK(hipMemcpyAsync(dz, z, sizeof(double) * 2 * n, hipMemcpyHostToDevice, stream_));
+	if (mu)
+		k_deflect_points<PREC_FP64, true><<<blocks_for(n, 256), 256, 0, stream_>>>(a, dz, n, dw, dmu);
+	else if (prec == PREC_FP64)
+		k_deflect_points<PREC_FP64, false><<<blocks_for(n, 256), 256, 0, stream_>>>(a, dz, n, dw, nullptr);
+	else
+		k_deflect_points<PREC_MIXED, false><<<blocks_for(n, 256), 256, 0, stream_>>>(a, dz, n, dw, nullptr);
+	MLENS_HIP_CHECK(hipGetLastError());
+	MLENS_HIP_CHECK(hipMemcpyAsync(w, dw, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, stream_));
+	if (mu) MLENS_HIP_CHECK(hipMemcpyAsync(mu, dmu, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+	MLENS_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 void Engine::irs_positions(const LensParams& lens, const GridParams& g, Precision prec, long long row, double* w)

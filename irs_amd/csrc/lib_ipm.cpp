@@ -40,9 +40,24 @@ bool ipm_finish(IPMHandle* self, int verbose) { MLENS_GUARD(return self->m.finis
 
 bool ipm_deflect(IPMHandle* self, const double* z, int n, double* w, double* mu)
 {
-	if (!self->m.engine()) return false;
+	if (!self->m.engine() || !self->m.is_built())
+	{
+		std::cerr << "Error. build() must succeed before ipm_deflect()\n";
+		return false;
+	}
 	MLENS_GUARD(self->m.engine()->deflect_points(self->m.lens(), z, n, mlens::Precision(self->m.precision), w, mu); return true,
 		return false)
+}
+
+bool ipm_deflect_noalloc(IPMHandle* self, const double* z, int n, double* w, double* mu)
+{
+	if (!self->m.engine() || !self->m.is_built())
+	{
+		std::cerr << "Error. build() must succeed before ipm_deflect_noalloc()\n";
+		return false;
+	}
+	MLENS_GUARD(self->m.engine()->deflect_points_noalloc(self->m.lens(), z, n, mlens::Precision(self->m.precision), w, mu);
+				return true, return false)
 }
 
 long long ipm_window_cells(IPMHandle* self, int x0, int y0, int width, int height, double margin, long long* records,

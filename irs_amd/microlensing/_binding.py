@@ -95,6 +95,7 @@ def load(path, dtype, irs=False):
         sig("ipm_shoot", [v, i, ll, ll], b)
         sig("ipm_finish", [v, i], b)
         sig("ipm_deflect", [v, P(ctypes.c_double), i, P(ctypes.c_double), P(ctypes.c_double)], b)
+        sig("ipm_deflect_noalloc", [v, P(ctypes.c_double), i, P(ctypes.c_double), P(ctypes.c_double)], b)
         sig("ipm_window_cells", [v, i, i, i, i, ctypes.c_double, P(ll), ll], ll)
         sig("ipm_export_tree", [v, P(ctypes.c_double), P(i), P(i)], b)
         sig("ipm_export_coeffs", [v, i, P(ctypes.c_double)], b)
@@ -125,7 +126,7 @@ IPM_SYMBOLS = (["init"] + [f"set_{n}" for n, _ in _FIELDS[:26]] + [f"get_{n}" fo
                 "set_kernel", "get_kernel", "get_tree_levels", "get_alpha_error", "get_num_ray_threads_1",
                 "get_num_ray_threads_2", "get_root_half_length", "get_mean_near_stars", "get_taylor_smooth"] + _DERIVED +
                ["ipm_prepare", "ipm_build",
-                "ipm_shoot", "ipm_finish", "get_cells_shot", "ipm_deflect", "ipm_window_cells", "get_num_nodes", "ipm_export_tree",
+                "ipm_shoot", "ipm_finish", "get_cells_shot", "ipm_deflect", "ipm_deflect_noalloc", "ipm_window_cells", "get_num_nodes", "ipm_export_tree",
                 "ipm_export_coeffs", "ipm_derive", "ipm_batch_create", "ipm_batch_free", "ipm_batch_add",
                 "ipm_batch_shoot", "get_batch_t_shoot", "get_batch_cells", "get_batch_size", "ipm_batch_histograms",
                 "get_batch_tile_h", "get_batch_blocks_per_cu", "get_batch_t_tune", "ipm_batch_set_tuning"] + _M2L_SYMBOLS)
