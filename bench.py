@@ -229,7 +229,8 @@ def surrogate_bench(n_train=20000, n_cal=1024, n_test=2048, epochs=150, seed=11)
     t_map = t_ipm / len(params)
     out = {"train": n_train, "calibration": n_cal, "test": n_test, "epochs": epochs, "t_data_s": t_data,
            "t_ipm_batched_s": t_ipm, "t_train_s": t_train, "loss_first": losses[0], "loss_last": losses[-1],
-           "entropy_threshold": float(thr), "accepted_fraction": float(acc.mean()),
+           "entropy_threshold": float(thr), "fallback_when_entropy": "above" if s.reject_above else "below",
+           "accepted_fraction": float(acc.mean()),
            "ks_median": float(np.median(ks)), "ks_p95": float(np.percentile(ks, 95)), "ks_max": float(ks.max()),
            "ks_accepted_median": float(np.median(ks[acc])) if acc.any() else None,
            "ks_accepted_p95": float(np.percentile(ks[acc], 95)) if acc.any() else None,
