@@ -220,17 +220,16 @@ image of a demagnified region near a star then no longer stalls the other 63
 (those images are a tenth of the general triangles and four fifths of their
 pixels).
 */
-constexpr int IPM_QCAP = 80;                 // entries per wave
+constexpr int IPM_QCAP = 79;                 // entries per wave
 constexpr int IPM_QFLUSH = IPM_QCAP - 64;    // an enqueue adds at most 64
 
 struct TriQueue
 {
 	double x[3][IPM_QCAP], y[3][IPM_QCAP];       // ccw vertices relative to (X0, Y0)
-	double idy[3][IPM_QCAP], k[3][IPM_QCAP];     // per edge: 1 / dy and dx / dy (0 when dy = 0)
+	double k[3][IPM_QCAP];                       // per edge: dx / dy (0 when dy = 0)
 	double scale[IPM_QCAP];                      // factor / triangle area
 	int X0[IPM_QCAP], Y0[IPM_QCAP];
-	int w[IPM_QCAP];                             // box width in pixels
-	int bit[IPM_QCAP];                           // map: 0 = total or minima, 1 = saddles
+	int wb[IPM_QCAP];                            // box width in pixels << 1 | map (0 = total or minima, 1 = saddles)
 	int end[IPM_QCAP];                           // box rows, then running total
 	int ring_ey[4 * 64];                         // edge-crossed pixels awaiting their exact area: entry | map bit << 7 | row << 8
 	int ring_x[4 * 64];                          //   and their map column
@@ -354,7 +353,7 @@ and a horizontal edge admits the whole row or none of it.
 */
 __device__ __forceinline__ void tri_queue_row(const TriQueue& q, int e, int yy, int& xl, int& xr, int& il, int& ir)
 {
-	const double wmax = double(q.w[e] - 1);
+	const double wmax = double((q.wb[e] >> 1) - 1);
 	const double Y = double(yy);
 	double lo = 0.0, hi = wmax, ilo = 0.0, ihi = wmax;
 #pragma unroll
@@ -362,7 +361,7 @@ __device__ __forceinline__ void tri_queue_row(const TriQueue& q, int e, int yy, 
 	{
 		const int n = (i + 1) % 3;
 		const double xi = q.x[i][e], yi = q.y[i][e];
-		const double dx = q.x[n][e] - xi, dy = q.y[n][e] - yi, idy = q.idy[i][e];
+		const double dx = q.x[n][e] - xi, dy = q.y[n][e] - yi, idy = dy != 0 ? dev::rcp64(dy) : 0.0;
 		const double tol = 1e-9 * (fabs(dx) + fabs(dy));
 		const double base = dy * xi - dx * yi;
 		const double A = fma(dx, dx > 0 ? Y : Y + 1.0, base);
@@ -476,7 +475,6 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 				const int n = (i + 1) % 3;
 				const double dy = y[n] - y[i];
 				const double idy = dy != 0 ? dev::rcp64(dy) : 0.0;
-				q.idy[i][e] = idy;
 				q.k[i][e] = (x[n] - x[i]) * idy;
 			}
 			q.scale[e] = factor * dev::rcp64(area2 * 0.5);
@@ -526,7 +524,7 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 				xa = X0 + xl;
 				ila = X0 + il;
 				ira = X0 + ir;
-				ey = e | (q.bit[e] << 7) | (yy << 8);
+				ey = e | ((q.wb[e] & 1) << 7) | (yy << 8);
 				rbase = map_row_base<MapT, TILED>(a, q.Y0[e] + yy);
 				full = MapT(q.scale[e]);
 			}
@@ -628,8 +626,7 @@ __device__ __forceinline__ void tri_queue_push(TriQueue& q, int& qn, bool gen, c
 		}
 		q.X0[pos] = g.X0;
 		q.Y0[pos] = g.Y0;
-		q.w[pos] = g.w;
-		q.bit[pos] = g.bit;
+		q.wb[pos] = (g.w << 1) | g.bit;
 		q.end[pos] = g.h;
 	}
 	qn += __popcll(m);
@@ -642,7 +639,7 @@ constexpr int IPM_WAVES = 1;
 constexpr int IPM_TILE_W = 63 * IPM_WAVES;
 constexpr int IPM_BATCH_WAVES = 1;
 constexpr int IPM_BATCH_TILE_W = 63 * IPM_BATCH_WAVES;
-constexpr int IPM_RESIDENT_WAVES_PER_CU = 12;   // 3 per SIMD at up to 168 VGPRs
+constexpr int IPM_RESIDENT_WAVES_PER_CU = 16;   // 4 per SIMD: 128 VGPRs and a 10 KB wave queue
 constexpr int IPM_TILE_H = 64;       // lattice rows a wave walks: the shortest tile
 constexpr int IPM_TILE_H_MAX = 1024;  // and the tallest
 
@@ -719,7 +716,7 @@ __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long l
 }
 
 template <int PREC, bool PARITY>
-__global__ __launch_bounds__(64 * IPM_WAVES) void k_ipm_fast(ShootArgs a, long long row0, long long row1,
+__global__ __launch_bounds__(64 * IPM_WAVES, 4) void k_ipm_fast(ShootArgs a, long long row0, long long row1,
 	MLENS_TABLE_PARAMS, void* __restrict__ map, void* __restrict__ mmin, void* __restrict__ msad, int tile_h)
 {
 	MLENS_TABLE_REBIND(a)
@@ -835,7 +832,7 @@ struct BatchMap
 	int n1, tiles_x;
 };
 
-__global__ __launch_bounds__(64 * IPM_BATCH_WAVES) void k_ipm_batch(const ShootArgs* __restrict__ args,
+__global__ __launch_bounds__(64 * IPM_BATCH_WAVES, 4) void k_ipm_batch(const ShootArgs* __restrict__ args,
 	const long long* __restrict__ tile_start, int ninst, int tile_h, long long total, long long stride,
 	unsigned long long* __restrict__ counter)
 {
@@ -1882,7 +1879,7 @@ struct BatchShooter::Impl
 	std::vector<long long> tiles_x;   // per instance: column tiles
 	std::vector<std::unique_ptr<DeviceBuffer<char>>> tiled;   // per instance: tiled accumulation map
 	bool dirty = true;
-	int tile_h = IPM_TILE_H, bpc = 12, ncu = 0;
+	int tile_h = IPM_TILE_H, bpc = 16, ncu = 0;
 	double tune_ms = 0;
 	bool tuned = false;
 	hipStream_t stream = nullptr;
@@ -1972,7 +1969,7 @@ struct BatchShooter::Impl
 		// about an eighth of the tiles, spread over every instance
 		const long long stride = 8;
 		const int heights[3] = {32, 64, 128};
-		const int resident[3] = {8, 10, 12};   // one-wave blocks; 12 fit at up to 168 VGPRs
+		const int resident[3] = {8, 12, 16};   // one-wave blocks; 16 fit at 128 VGPRs and 10 KB of LDS
 		float best = -1;
 		auto t0 = std::chrono::steady_clock::now();
 		for (int h : heights)
