@@ -832,7 +832,10 @@ struct BatchMap
 	int n1, tiles_x;
 };
 
-__global__ __launch_bounds__(64 * IPM_BATCH_WAVES, 4) void k_ipm_batch(const ShootArgs* __restrict__ args,
+// Three waves per SIMD (138 VGPRs, no spills): the batch's maps of 0.25-16 MB
+// stay in the caches, and at four waves (128 VGPRs) the spills cost more than
+// the fourth wave hides (256 maps: 1.887 s at 12 blocks per CU, 2.107 s at 16)
+__global__ __launch_bounds__(64 * IPM_BATCH_WAVES, 3) void k_ipm_batch(const ShootArgs* __restrict__ args,
 	const long long* __restrict__ tile_start, int ninst, int tile_h, long long total, long long stride,
 	unsigned long long* __restrict__ counter)
 {
@@ -1879,7 +1882,7 @@ struct BatchShooter::Impl
 	std::vector<long long> tiles_x;   // per instance: column tiles
 	std::vector<std::unique_ptr<DeviceBuffer<char>>> tiled;   // per instance: tiled accumulation map
 	bool dirty = true;
-	int tile_h = IPM_TILE_H, bpc = 16, ncu = 0;
+	int tile_h = IPM_TILE_H, bpc = 12, ncu = 0;
 	double tune_ms = 0;
 	bool tuned = false;
 	hipStream_t stream = nullptr;
@@ -1969,7 +1972,7 @@ struct BatchShooter::Impl
 		// about an eighth of the tiles, spread over every instance
 		const long long stride = 8;
 		const int heights[3] = {32, 64, 128};
-		const int resident[3] = {8, 12, 16};   // one-wave blocks; 16 fit at 128 VGPRs and 10 KB of LDS
+		const int resident[3] = {8, 10, 12};   // one-wave blocks; 12 fill a CU at 3 waves per SIMD
 		float best = -1;
 		auto t0 = std::chrono::steady_clock::now();
 		for (int h : heights)

@@ -57,10 +57,10 @@ def _build(m):
     m.build()
 
 
-def histograms(params, chunk=256, threads=8, tuning=(64, 16), log=None):
+def histograms(params, chunk=256, threads=8, tuning=(64, 12), log=None):
     """(n, 128) histogram counts of the parameter draws, and the seconds spent
     in the batched launches. `tuning` fixes the batch's tile height and its
-    resident one-wave blocks per CU (16 fill a CU) instead of autotuning."""
+    resident one-wave blocks per CU (12 fill a CU) instead of autotuning."""
     out = np.zeros((len(params), 128), dtype=np.uint32)
     t_shoot = 0.0
     with ThreadPoolExecutor(threads) as pool:
