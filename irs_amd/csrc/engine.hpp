@@ -225,6 +225,8 @@ private:
 	DeviceBuffer<char> map_, map_min_, map_sad_;
 	// tiled accumulation maps of the fast IPM kernel (all zero between shots)
 	DeviceBuffer<char> tmap_, tmap_min_, tmap_sad_;
+	// FP64 span maps beside them: span_ for the map, span_min_ / span_sad_ with parities
+	DeviceBuffer<char> span_, span_min_, span_sad_;
 	int tiles_x_ = 0;
 	size_t tmap_bytes_ = 0;
 	void ensure_tiled_maps();

@@ -136,6 +136,46 @@ __device__ __forceinline__ long long tiled_index(int tiles_x, int X, int Y)
 	return tiled_row_base<MapT>(tiles_x, Y) + tiled_col<MapT>(X);
 }
 
+/*
+One row y of a map after the shot: map[y][x] += tiled[y][x] + s (= with ASSIGN), where s is the
+running FP64 sum of span[y][0..x] (the span map has the tiled map's type and
+layout; it may be null), and the tiled and span maps are cleared. A pixel that
+no fully covered run reaches has s = 0 up to rounding, and every run weighs
+more than 1e-4 (a triangle's weight per pixel is factor / area, and the kernel
+drops triangles of area 1e4 * factor or more), while the residue is of order
+2^-24 (FP32 spans) times the weights summed into the span cells before it along
+the row; so |s| < 2^-16 is such a pixel, and it gets exactly nothing.
+*/
+template <typename MapT, bool ASSIGN>
+__device__ __forceinline__ void fold_row(MapT* __restrict__ row, MapT* __restrict__ tiled, MapT* __restrict__ span, int n1,
+	int tiles_x, int y)
+{
+	const int lane = threadIdx.x & 63;
+	const long long rt = tiled_row_base<MapT>(tiles_x, y);
+	double run = 0.0;
+	for (int x0 = 0; x0 < n1; x0 += 64)
+	{
+		const int x = x0 + lane;
+		const bool in = x < n1;
+		const long long it = rt + tiled_col<MapT>(x);
+		const MapT dm = (in && span) ? span[it] : MapT(0);
+		const double d = double(dm);
+		const MapT v = in ? tiled[it] : MapT(0);
+		const double s = run + wave_prefix_sum(d);
+		run = __shfl(s, 63);
+		if (in)
+		{
+			const MapT add = v + (fabs(s) < 0x1p-16 ? MapT(0) : MapT(s));
+			if (ASSIGN)
+				row[x] = add;
+			else if (add != MapT(0))
+				row[x] += add;
+			if (v != MapT(0)) tiled[it] = MapT(0);
+			if (dm != MapT(0)) span[it] = MapT(0);
+		}
+	}
+}
+
 // the same for the map a kernel writes: tiled (TILED) or row-major
 template <typename MapT, bool TILED>
 __device__ __forceinline__ long long map_row_base(const ShootArgs& a, int Y)
@@ -403,6 +443,23 @@ __device__ __forceinline__ void map_add(const ShootArgs& a, long long pix, int b
 	unsafeAtomicAdd(map + pix, v);
 }
 
+/*
+A fully covered run [x0, x1] of pixel row Y gets weight w in every pixel. It is
+kept as two differences in a span map of the map's type and tiles, +w at x0 and
+-w at x1 + 1 (none past the map's last column); the fold adds each row's running
+sum, taken in FP64, to the map. A run costs two atomics instead of one per
+pixel, and a tile's line holds the run ends of 8 (FP32) or 4 (FP64) rows.
+*/
+template <typename MapT, bool PARITY>
+__device__ __forceinline__ void span_add(const ShootArgs& a, int Y, int x0, int x1, int bit, double w)
+{
+	MapT* span = static_cast<MapT*>((PARITY && bit) ? a.span_sad : a.span);
+	const long long rb = tiled_row_base<MapT>(a.tiles_x, Y);
+	const MapT v = MapT(w);
+	unsafeAtomicAdd(span + rb + tiled_col<MapT>(x0), v);
+	if (x1 + 1 < a.n1) unsafeAtomicAdd(span + rb + tiled_col<MapT>(x1 + 1), -v);
+}
+
 // exact areas (FP64, Green's theorem over the pixel's row) of ring slots [start, start + count), one per lane
 template <typename MapT, bool PARITY, bool TILED>
 __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q, int start, int count)
@@ -429,14 +486,14 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 /*
 Rasterise every queued triangle. Rows of the concatenated boxes go 64 at a
 time to the lanes, which find their row's candidate and fully covered
-columns; the candidate pixels of those 64 rows then go to the lanes, 64 at a
-time. A lane finds its entry (or row) as the last one that starts at or
-before its own index: every entry (row) starting in the current pass marks
-its first slot in LDS, and a max-scan over the slots carries each mark
-forward, starting from the entry (row) the previous pass ended in. A fully
-covered pixel gets the triangle's whole per-pixel weight; a crossed one goes
-to the ring and gets its exact area in full passes of 64. rh / rt: next ring
-slot to process / to fill.
+columns and hand the fully covered run to the span map (span_add); the
+remaining candidate pixels of those 64 rows, the ones an edge may cross, then
+go to the lanes, 64 at a time, and from there to the ring, where they get
+their exact areas in full passes of 64. A lane finds its entry (or row) as the
+last one that starts at or before its own index: every entry (row) starting in
+the current pass marks its first slot in LDS, and a max-scan over the slots
+carries each mark forward, starting from the entry (row) the previous pass
+ended in. rh / rt: next ring slot to process / to fill.
 */
 template <typename MapT, bool PARITY, bool TILED>
 __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q, int qn)
@@ -508,12 +565,10 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 		int e = max(wave_max_scan(q.mark[lane]), ecarry);
 		ecarry = __builtin_amdgcn_readlane(e, 63);
 		e = min(e, qn - 1);
-		// this lane's row: candidate columns from xa (absolute), cnt of them, fully
-		// covered ones [ila, ira], the map index of the row, the weight of a full pixel
+		// this lane's row: its candidate columns, cnt of them, are the cl columns
+		// from xa (absolute) and the rest after a gap (the fully covered run)
 		const int r = r0 + lane;
-		int xa = 0, cnt = 0, ila = 0, ira = -1, ey = 0;
-		long long rbase = 0;
-		MapT full = 0;
+		int xa = 0, cnt = 0, cl = 0, gap = 0, ey = 0;
 		if (r < total)
 		{
 			const int start = e > 0 ? q.end[e - 1] : 0;
@@ -522,59 +577,55 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 			{
 				int xl, xr, il, ir;
 				tri_queue_row(q, e, yy, xl, xr, il, ir);
+				const int X0 = q.X0[e], bit = q.wb[e] & 1;
 				cnt = max(xr - xl + 1, 0);
-				const int X0 = q.X0[e];
+				cl = cnt;
+				il = max(il, xl);
+				ir = min(ir, xr);
+				if (il <= ir)
+				{
+					span_add<MapT, PARITY>(a, q.Y0[e] + yy, X0 + il, X0 + ir, bit, q.scale[e]);
+					cl = il - xl;
+					gap = ir - il + 1;
+					cnt -= gap;
+				}
 				xa = X0 + xl;
-				ila = X0 + il;
-				ira = X0 + ir;
-				ey = e | ((q.wb[e] & 1) << 7) | (yy << 8);
-				rbase = map_row_base<MapT, TILED>(a, q.Y0[e] + yy);
-				full = MapT(q.scale[e]);
+				ey = e | (bit << 7) | (yy << 8);
 			}
 		}
 		int off = wave_prefix_sum(cnt);
 		const int tot = __builtin_amdgcn_readlane(off, 63);
 		const int lim = off;   // candidate k of this row: off - cnt <= k < lim
 		off -= cnt;
-		const int xbase = xa - off;
+		const int xbase = xa - off;   // column of candidate k: xbase + k, plus gap from k = thr on
+		const int thr = off + cl;
 		int jcarry = 0;
 		// candidate pixel k of row j: the row's data, fetched from the row's lane
 		struct Cand
 		{
-			int k, x, lim, ey, ila, ira;
-			long long rb;
-			MapT full;
+			int k, x, lim, ey;
 		};
 		auto fetch = [&](int k, int j) {
+			// every lane takes part in every shuffle: a lane shuffle reads nothing
+			// from a lane that is inactive, so none may sit in a conditional arm
+			const int xj = __shfl(xbase, j), tj = __shfl(thr, j), gj = __shfl(gap, j);
 			Cand c;
 			c.k = k;
-			c.x = __shfl(xbase, j) + k;
+			c.x = xj + k + (k >= tj ? gj : 0);
 			c.lim = __shfl(lim, j);
 			c.ey = __shfl(ey, j);
-			c.ila = __shfl(ila, j);
-			c.ira = __shfl(ira, j);
-			c.rb = __shfl(rbase, j);
-			c.full = __shfl(full, j);
 			return c;
 		};
-		// a fully covered pixel gets the row's whole weight, a crossed one joins the ring
+		// every candidate joins the ring
 		auto pixel = [&](const Cand& c) {
-			const int x = c.x, rey = c.ey;
-			bool crossed = false;
-			if (c.k < tot && c.k < c.lim)
-			{
-				if (x >= c.ila && x <= c.ira)
-					map_add<MapT, PARITY>(a, c.rb + map_col<MapT, TILED>(x), (rey >> 7) & 1, c.full);
-				else
-					crossed = true;
-			}
+			const bool crossed = c.k < tot && c.k < c.lim;
 			const unsigned long long m = __ballot(crossed);
 			if (crossed)
 			{
 				int pos = rt + int(__builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u)));
 				asm volatile("" : "+v"(pos));
-				q.ring_ey[pos & 255] = rey;
-				q.ring_x[pos & 255] = x;
+				q.ring_ey[pos & 255] = c.ey;
+				q.ring_x[pos & 255] = c.x;
 			}
 			rt += __popcll(m);
 		};
@@ -720,12 +771,15 @@ __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long l
 
 template <int PREC, bool PARITY>
 __global__ __launch_bounds__(64 * IPM_WAVES, 4) void k_ipm_fast(ShootArgs a, long long row0, long long row1,
-	MLENS_TABLE_PARAMS, void* __restrict__ map, void* __restrict__ mmin, void* __restrict__ msad, int tile_h)
+	MLENS_TABLE_PARAMS, void* __restrict__ map, void* __restrict__ mmin, void* __restrict__ msad,
+	void* __restrict__ span, void* __restrict__ span_sad, int tile_h)
 {
 	MLENS_TABLE_REBIND(a)
 	a.map = map;
 	a.mmin = mmin;
 	a.msad = msad;
+	a.span = span;
+	a.span_sad = span_sad;
 	const long long J0 = row0 + (long long)blockIdx.y * tile_h;
 	if (J0 >= row1) return;
 	const long long J1 = min(J0 + tile_h, row1);
@@ -830,9 +884,10 @@ struct BatchMap
 {
 	float* map;         // row-major n2 x n1, written by the fold
 	float* tiled;       // tiled accumulation map the shooting kernel writes (MapTile<float>)
+	float* span;        // its span map (span_add), laid out as the tiled map
 	long long npix;     // n1 * n2
-	long long ntiled;   // elements of the tiled map
-	int n1, tiles_x;
+	long long ntiled;   // elements of the tiled map and of the span map
+	int n1, n2, tiles_x;
 };
 
 // Three waves per SIMD (138 VGPRs, no spills): the batch's maps of 0.25-16 MB
@@ -869,7 +924,7 @@ __global__ __launch_bounds__(64 * IPM_BATCH_WAVES, 3) void k_ipm_batch(const Sho
 	}
 }
 
-// zero every instance's tiled map (blockIdx.y = instance), its histogram and the work counter
+// zero every instance's tiled and span maps (blockIdx.y = instance), its histogram and the work counter
 __global__ void k_batch_zero(const BatchMap* __restrict__ maps, unsigned* hist, unsigned long long* counter)
 {
 	const int inst = blockIdx.y;
@@ -880,20 +935,19 @@ __global__ void k_batch_zero(const BatchMap* __restrict__ maps, unsigned* hist, 
 	}
 	const BatchMap m = maps[inst];
 	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m.ntiled; i += (long long)gridDim.x * blockDim.x)
+	{
 		m.tiled[i] = 0.0f;
+		m.span[i] = 0.0f;
+	}
 }
 
-// every instance's row-major map from its tiled map (blockIdx.y = instance); the tiled map is cleared
-__global__ void k_batch_fold(const BatchMap* __restrict__ maps)
+// every instance's row-major map from its tiled and span maps (blockIdx.y = instance, one wave per
+// row, blocks of four rows striding over the instance's rows); both are cleared
+__global__ __launch_bounds__(256) void k_batch_fold(const BatchMap* __restrict__ maps)
 {
 	const BatchMap m = maps[blockIdx.y];
-	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m.npix; i += (long long)gridDim.x * blockDim.x)
-	{
-		const int y = int(i / m.n1), x = int(i - (long long)y * m.n1);
-		const long long t = tiled_index<float>(m.tiles_x, x, y);
-		m.map[i] = m.tiled[t];
-		m.tiled[t] = 0.0f;
-	}
+	for (int y = blockIdx.x * 4 + (threadIdx.x >> 6); y < m.n2; y += gridDim.x * 4)
+		fold_row<float, true>(m.map + (long long)m.n1 * y, m.tiled, m.span, m.n1, m.tiles_x, y);
 }
 
 // 128-bin histogram of log10(magnification) over [-2, 2] per instance; values
@@ -1405,60 +1459,70 @@ void Engine::alloc_maps(int n1, int n2, bool ipm, Precision prec, bool parities)
 
 /*
 Tiled accumulation maps for the fast IPM kernel: 128-byte tiles of
-MapTile<MapT>::W x H pixels, ceil(n1 / W) tiles per row. Allocated zeroed on
-first use (and again when the map's shape or type changes); fold_tiled_maps
-adds them into the row-major maps and clears them, so they are zero between
-shots.
+MapTile<MapT>::W x H pixels, ceil(n1 / W) tiles per row, and beside each an
+span map of the same type and layout (span_add). Allocated zeroed on first
+use (and again when the map's shape or type changes); fold_tiled_maps adds
+them into the row-major maps and clears them, so they are zero between shots.
 */
 void Engine::ensure_tiled_maps()
 {
 	const int W = map_fp32_ ? MapTile<float>::W : MapTile<double>::W, H = map_fp32_ ? MapTile<float>::H : MapTile<double>::H;
 	const int tx = (npix_[0] + W - 1) / W;
 	const size_t bytes = size_t(tx) * size_t((npix_[1] + H - 1) / H) * 128;
-	if (tmap_.ptr && tiles_x_ == tx && tmap_bytes_ == bytes && (!parities_ || tmap_min_.ptr)) return;
+	const size_t sbytes = bytes;
+	const bool have = parities_ ? (tmap_min_.ptr && span_min_.ptr && span_sad_.ptr) : span_.ptr != nullptr;
+	if (tmap_.ptr && tiles_x_ == tx && tmap_bytes_ == bytes && have) return;
 	tiles_x_ = tx;
 	tmap_bytes_ = bytes;
 	tmap_.alloc(bytes);
 	MLENS_HIP_CHECK(hipMemsetAsync(tmap_.ptr, 0, bytes, stream_));
 	tmap_min_.free();
 	tmap_sad_.free();
+	span_.free();
+	span_min_.free();
+	span_sad_.free();
 	if (parities_)
 	{
 		tmap_min_.alloc(bytes);
 		tmap_sad_.alloc(bytes);
+		span_min_.alloc(sbytes);
+		span_sad_.alloc(sbytes);
 		MLENS_HIP_CHECK(hipMemsetAsync(tmap_min_.ptr, 0, bytes, stream_));
 		MLENS_HIP_CHECK(hipMemsetAsync(tmap_sad_.ptr, 0, bytes, stream_));
+		MLENS_HIP_CHECK(hipMemsetAsync(span_min_.ptr, 0, sbytes, stream_));
+		MLENS_HIP_CHECK(hipMemsetAsync(span_sad_.ptr, 0, sbytes, stream_));
+	}
+	else
+	{
+		span_.alloc(sbytes);
+		MLENS_HIP_CHECK(hipMemsetAsync(span_.ptr, 0, sbytes, stream_));
 	}
 }
 
-// map[y][x] += tiled[y][x] and tiled[y][x] = 0; a block of 64 x 4 threads covers 64 columns of 4 rows
+// the fold of a whole map: a block of 4 waves takes 4 rows, one wave per row
 template <typename MapT>
-__global__ __launch_bounds__(256) void k_fold_tiles(MapT* __restrict__ map, MapT* __restrict__ tiled, int n1, int n2, int tiles_x)
+__global__ __launch_bounds__(256) void k_fold_rows(MapT* __restrict__ map, MapT* __restrict__ tiled, MapT* __restrict__ span,
+	int n1, int n2, int tiles_x)
 {
-	const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-	const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-	if (x >= n1 || y >= n2) return;
-	const long long t = tiled_index<MapT>(tiles_x, x, y);
-	const MapT v = tiled[t];
-	if (v != MapT(0))
-	{
-		map[(long long)n1 * y + x] += v;
-		tiled[t] = MapT(0);
-	}
+	const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (y >= n2) return;
+	fold_row<MapT, false>(map + (long long)n1 * y, tiled, span, n1, tiles_x, y);
 }
 
 void Engine::fold_tiled_maps()
 {
 	if (!tmap_.ptr) return;
-	dim3 grid(unsigned((npix_[0] + 63) / 64), unsigned((npix_[1] + 3) / 4));
+	const unsigned grid = unsigned((npix_[1] + 3) / 4);
 	for (int k = 0; k < (parities_ ? 3 : 1); k++)
 	{
 		char* m = k == 0 ? map_.ptr : (k == 1 ? map_min_.ptr : map_sad_.ptr);
 		char* t = k == 0 ? tmap_.ptr : (k == 1 ? tmap_min_.ptr : tmap_sad_.ptr);
+		// with parities the kernel writes minima and saddles only, so the total has no spans
+		char* sp = k == 0 ? (parities_ ? nullptr : span_.ptr) : (k == 1 ? span_min_.ptr : span_sad_.ptr);
 		if (map_fp32_)
-			k_fold_tiles<float><<<grid, 256, 0, stream_>>>((float*)m, (float*)t, npix_[0], npix_[1], tiles_x_);
+			k_fold_rows<float><<<grid, 256, 0, stream_>>>((float*)m, (float*)t, (float*)sp, npix_[0], npix_[1], tiles_x_);
 		else
-			k_fold_tiles<double><<<grid, 256, 0, stream_>>>((double*)m, (double*)t, npix_[0], npix_[1], tiles_x_);
+			k_fold_rows<double><<<grid, 256, 0, stream_>>>((double*)m, (double*)t, (double*)sp, npix_[0], npix_[1], tiles_x_);
 	}
 	MLENS_HIP_CHECK(hipGetLastError());
 }
@@ -1473,8 +1537,8 @@ void Engine::zero_maps()
 		MLENS_HIP_CHECK(hipMemsetAsync(map_min_.ptr, 0, bytes, stream_));
 		MLENS_HIP_CHECK(hipMemsetAsync(map_sad_.ptr, 0, bytes, stream_));
 	}
-	// tiled maps are zero between shots unless a shot was cut short
-	for (DeviceBuffer<char>* t : {&tmap_, &tmap_min_, &tmap_sad_})
+	// tiled and span maps are zero between shots unless a shot was cut short
+	for (DeviceBuffer<char>* t : {&tmap_, &tmap_min_, &tmap_sad_, &span_, &span_min_, &span_sad_})
 		if (t->ptr) MLENS_HIP_CHECK(hipMemsetAsync(t->ptr, 0, std::min(tmap_bytes_, t->count), stream_));
 }
 
@@ -1512,6 +1576,8 @@ ShootStats Engine::shoot_ipm(const LensParams& lens, const GridParams& g, Precis
 		a.map = tmap_.ptr;
 		a.mmin = parities_ ? tmap_min_.ptr : nullptr;
 		a.msad = parities_ ? tmap_sad_.ptr : nullptr;
+		a.span = parities_ ? span_min_.ptr : span_.ptr;
+		a.span_sad = parities_ ? span_sad_.ptr : nullptr;
 		const long long ntx = (a.N1 + IPM_TILE_W - 1) / IPM_TILE_W;
 		const int th = tile_rows_ > 0 ? tile_rows_ : ipm_tile_rows(ntx, row1 - row0, num_cus_);
 		last_tile_rows_ = th;
@@ -1520,13 +1586,17 @@ ShootStats Engine::shoot_ipm(const LensParams& lens, const GridParams& g, Precis
 		dim3 block(64 * IPM_WAVES);
 		if (prec == PREC_FP64)
 		{
-			if (parities_) k_ipm_fast<PREC_FP64, true><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad, th);
-			else k_ipm_fast<PREC_FP64, false><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad, th);
+			if (parities_) k_ipm_fast<PREC_FP64, true><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad,
+				a.span, a.span_sad, th);
+			else k_ipm_fast<PREC_FP64, false><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad,
+				a.span, a.span_sad, th);
 		}
 		else
 		{
-			if (parities_) k_ipm_fast<PREC_MIXED, true><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad, th);
-			else k_ipm_fast<PREC_MIXED, false><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad, th);
+			if (parities_) k_ipm_fast<PREC_MIXED, true><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad,
+				a.span, a.span_sad, th);
+			else k_ipm_fast<PREC_MIXED, false><<<grid, block, 0, stream_>>>(a, row0, row1, MLENS_TABLE_ARGS(a), a.map, a.mmin, a.msad,
+				a.span, a.span_sad, th);
 		}
 		MLENS_HIP_CHECK(hipGetLastError());
 		fold_tiled_maps();
@@ -1883,7 +1953,7 @@ struct BatchShooter::Impl
 	std::vector<ShootArgs> args;
 	std::vector<BatchMap> maps;
 	std::vector<long long> tiles_x;   // per instance: column tiles
-	std::vector<std::unique_ptr<DeviceBuffer<char>>> tiled;   // per instance: tiled accumulation map
+	std::vector<std::unique_ptr<DeviceBuffer<char>>> tiled;   // per instance: tiled accumulation and span maps
 	bool dirty = true;
 	int tile_h = IPM_TILE_H, bpc = 12, ncu = 0;
 	double tune_ms = 0;
@@ -1899,6 +1969,7 @@ struct BatchShooter::Impl
 	DeviceBuffer<int> d_hist;
 	long long total = 0;
 	long long max_npix = 0;
+	int max_rows = 0;
 
 	long long tiles(int h) const
 	{
@@ -1948,7 +2019,7 @@ struct BatchShooter::Impl
 
 	void launch_fold()
 	{
-		const unsigned bx = unsigned(std::min<long long>(1024, std::max<long long>(1, (max_npix + 255) / 256)));
+		const unsigned bx = unsigned(std::min(1024, std::max(1, (max_rows + 3) / 4)));
 		dim3 grid(bx, unsigned(args.size()));
 		k_batch_fold<<<grid, 256, 0, stream>>>(reinterpret_cast<const BatchMap*>(d_maps.ptr));
 		MLENS_HIP_CHECK(hipGetLastError());
@@ -2047,23 +2118,31 @@ int BatchShooter::add(Engine* e, const LensParams& lens, const GridParams& g)
 	ShootArgs a;
 	e->fill_batch_args(&a, lens, g);
 	if (a.N1 <= 0 || a.N2 <= 0) throw HipError("a batched instance has an empty lattice", __FILE__, __LINE__);
-	// the kernel accumulates into a tiled map of its own; the fold writes the row-major map
+	// the kernel accumulates into a tiled map and a span map of its own; the fold writes the row-major map
 	const int W = MapTile<float>::W, H = MapTile<float>::H;
 	const int tx = (a.n1 + W - 1) / W;
 	const size_t tbytes = size_t(tx) * size_t((a.n2 + H - 1) / H) * 128;
+	const size_t sbytes = tbytes;
 	auto buf = std::make_unique<DeviceBuffer<char>>();
 	buf->alloc(tbytes);
 	MLENS_HIP_CHECK(hipMemset(buf->ptr, 0, tbytes));
-	BatchMap bm{reinterpret_cast<float*>(a.map), reinterpret_cast<float*>(buf->ptr), (long long)a.n1 * a.n2,
-		(long long)(tbytes / sizeof(float)), a.n1, tx};
+	auto sbuf = std::make_unique<DeviceBuffer<char>>();
+	sbuf->alloc(sbytes);
+	MLENS_HIP_CHECK(hipMemset(sbuf->ptr, 0, sbytes));
+	BatchMap bm{reinterpret_cast<float*>(a.map), reinterpret_cast<float*>(buf->ptr), reinterpret_cast<float*>(sbuf->ptr),
+		(long long)a.n1 * a.n2, (long long)(tbytes / sizeof(float)), a.n1, a.n2, tx};
 	a.map = buf->ptr;
+	a.span = sbuf->ptr;
+	a.span_sad = nullptr;
 	a.tiles_x = tx;
 	impl_->tiled.push_back(std::move(buf));
+	impl_->tiled.push_back(std::move(sbuf));
 	impl_->engines.push_back(e);
 	impl_->args.push_back(a);
 	impl_->maps.push_back(bm);
 	impl_->tiles_x.push_back((a.N1 + IPM_BATCH_TILE_W - 1) / IPM_BATCH_TILE_W);
 	impl_->max_npix = std::max(impl_->max_npix, (long long)a.n1 * a.n2);
+	impl_->max_rows = std::max(impl_->max_rows, a.n2);
 	impl_->dirty = true;
 	impl_->graph_tile_h = -1;
 	return int(impl_->args.size()) - 1;

@@ -45,6 +45,10 @@ struct ShootArgs
 	const float* poly_f;
 	// maps
 	void *map, *mmin, *msad;
+	// the fast IPM kernel's fully covered row runs, as differences along each
+	// row (tiled maps of the map's type): span (the map, or minima with
+	// parities), span_sad
+	void *span, *span_sad;
 };
 
 // The hot read-only tables travel as separate __restrict__ kernel parameters:

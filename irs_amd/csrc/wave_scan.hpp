@@ -24,6 +24,28 @@ __device__ __forceinline__ int wave_prefix_sum(int v)
 	return v;
 }
 
+// one DPP step on an FP64 value (both halves moved alike; identity 0.0)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+	const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+	const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b)), CTRL, ROW_MASK, 0xf, false));
+	const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), CTRL, ROW_MASK, 0xf, false));
+	return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// the same inclusive sum in FP64
+__device__ __forceinline__ double wave_prefix_sum(double v)
+{
+	v += dpp_f64<0x111, 0xf>(v);   // row_shr:1
+	v += dpp_f64<0x112, 0xf>(v);   // row_shr:2
+	v += dpp_f64<0x114, 0xf>(v);   // row_shr:4
+	v += dpp_f64<0x118, 0xf>(v);   // row_shr:8
+	v += dpp_f64<0x142, 0xa>(v);   // row_bcast:15
+	v += dpp_f64<0x143, 0xc>(v);   // row_bcast:31
+	return v;
+}
+
 // the same for the maximum of values >= -1
 __device__ __forceinline__ int wave_max_scan(int v)
 {
