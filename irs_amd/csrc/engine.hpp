@@ -211,7 +211,7 @@ private:
 	int num_leaves_ = 0;
 	DeviceBuffer<int> leaf_of_node_;
 	DeviceBuffer<double> leaf_geom_;      // cx, cy, hl, 1/hl per leaf
-	DeviceBuffer<int> leaf_near_n_;       // padded to a multiple of 4
+	DeviceBuffer<int> leaf_near_n_;       // padded to a multiple of 4; FP32 polynomial entries to skip << 16
 	DeviceBuffer<int> leaf_near_true_;    // stars actually in the list
 	DeviceBuffer<double> leaf_near_d_;    // x, y, m (FP64), stride 32 per leaf
 	DeviceBuffer<float> leaf_near_f_;     // x - cx, y - cy, m (FP32), stride 32 per leaf

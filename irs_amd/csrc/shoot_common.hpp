@@ -119,7 +119,8 @@ __device__ __forceinline__ Deflection deflect_leaf(const ShootArgs& a, int L, do
 {
 	const double* g = a.leaf_geom + 4 * (long long)L;
 	const double cx = g[0], cy = g[1], ih = g[3];
-	const int nn = a.leaf_near_n[L];
+	const int packed = a.leaf_near_n[L];   // near stars (padded to 4) | FP32 entries to skip << 16
+	const int nn = packed & 0xffff;
 	const int p = a.p;
 	const double dzx = (zx - cx) * ih, dzy = (zy - cy) * ih;
 	double ax = 0, ay = 0, px = 0, py = 0;
@@ -175,7 +176,7 @@ __device__ __forceinline__ Deflection deflect_leaf(const ShootArgs& a, int L, do
 		const int SF = poly_stride_f(p);
 		const float* A = a.poly_f + (long long)L * SF * 2;
 		const float fdx = float(dzx), fdy = float(dzy);
-		for (int j0 = 0; j0 < SF; j0 += 4)
+		for (int j0 = packed >> 16; j0 < SF; j0 += 4)
 		{
 #pragma unroll
 			for (int j = j0; j < j0 + 4; j++)
