@@ -460,14 +460,27 @@ __device__ __forceinline__ void span_add(const ShootArgs& a, int Y, int x0, int 
 	if (x1 + 1 < a.n1) unsafeAtomicAdd(span + rb + tiled_col<MapT>(x1 + 1), -v);
 }
 
-// exact areas (FP64, Green's theorem over the pixel's row) of ring slots [start, start + count), one per lane
+/*
+Exact areas (FP64, Green's theorem over the pixel's row) of ring slots
+[start, start + count), one per lane. One pass holds 64 crossed pixels of a few
+neighbouring triangles, and a sixth of them (10000^2 map) land on a pixel that
+another lane of the pass also hits: small triangles around one pixel corner,
+and the pixels along edges two triangles share. Atomics on one address
+serialise, so those lanes are summed in LDS first: each lane writes its lane
+number into a hash slot of its pixel (the flush's mark array is free here),
+lanes whose slot holder has the same pixel add into the slot's cell, and only
+the holders and the lanes that lost their slot to another pixel issue atomics.
+64 slots of FP32 cells, or 32 of FP64, fill the mark array's 512 bytes.
+*/
 template <typename MapT, bool PARITY, bool TILED>
 __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q, int start, int count)
 {
 	const int lane = threadIdx.x & 63;
-	if (lane >= count) return;
 	int slot = start + lane;
 	asm volatile("" : "+v"(slot));
+	// every lane takes part in the merge; lanes past count read stale ring
+	// slots (entries and rows in range of the queue's arrays) and add nothing
+	const bool valid = lane < count;
 	const int ey = q.ring_ey[slot & 255], X = q.ring_x[slot & 255];
 	const int e = ey & 127, yy = ey >> 8;
 	double x[3], y[3], k[3];
@@ -479,8 +492,27 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 		k[i] = q.k[i][e];
 	}
 	const double ar = dev::tri_pixel_area(x, y, k, double(X - q.X0[e]), double(yy));
-	if (ar > 0)
-		map_add<MapT, PARITY>(a, map_index<MapT, TILED>(a, X, q.Y0[e] + yy), (ey >> 7) & 1, MapT(q.scale[e] * ar));
+	const bool live = valid && ar > 0;
+	const int bit = (ey >> 7) & 1;
+	const long long pix = live ? map_index<MapT, TILED>(a, X, q.Y0[e] + yy) : 0;
+	MapT v = MapT(q.scale[e] * ar);
+	constexpr int SLOTS = sizeof(MapT) == 4 ? 64 : 32;
+	static_assert(SLOTS * (1 + int(sizeof(MapT)) / 4) <= 2 * 64, "hash slots and cells exceed the mark array");
+	int* holder = q.mark;
+	MapT* cell = reinterpret_cast<MapT*>(q.mark + SLOTS);
+	const unsigned key = unsigned(pix) * 2u + unsigned(bit);
+	const int h = int((key ^ (key >> 6) ^ (key >> 12)) & unsigned(SLOTS - 1));
+	if (live) holder[h] = lane;
+	wave_lds_sync();
+	const int w = live ? holder[h] : lane;
+	const unsigned kw = __shfl(key, w);   // outside any condition: a shuffle reads nothing from inactive lanes
+	const bool merged = live && w != lane && kw == key;
+	if (live && w == lane) cell[h] = v;
+	wave_lds_sync();
+	if (merged) atomicAdd(&cell[h], v);
+	wave_lds_sync();
+	if (live && w == lane) v = cell[h];
+	if (live && !merged) map_add<MapT, PARITY>(a, pix, bit, v);
 }
 
 /*
