@@ -467,10 +467,12 @@ neighbouring triangles, and a sixth of them (10000^2 map) land on a pixel that
 another lane of the pass also hits: small triangles around one pixel corner,
 and the pixels along edges two triangles share. Atomics on one address
 serialise, so those lanes are summed in LDS first: each lane writes its lane
-number into a hash slot of its pixel (the flush's mark array is free here),
+number into the slot of its pixel (the flush's mark array is free here),
 lanes whose slot holder has the same pixel add into the slot's cell, and only
 the holders and the lanes that lost their slot to another pixel issue atomics.
-64 slots of FP32 cells, or 32 of FP64, fill the mark array's 512 bytes.
+64 slots of FP32 cells, or 32 of FP64, fill the mark array's 512 bytes. On the
+10000^2 map this took the mixed kernel from 2.062 to 1.744 s; a second round
+for the lanes that lost their slot made it slower (1.92 s).
 */
 template <typename MapT, bool PARITY, bool TILED>
 __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q, int start, int count)
@@ -501,7 +503,8 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	int* holder = q.mark;
 	MapT* cell = reinterpret_cast<MapT*>(q.mark + SLOTS);
 	const unsigned key = unsigned(pix) * 2u + unsigned(bit);
-	const int h = int((key ^ (key >> 6) ^ (key >> 12)) & unsigned(SLOTS - 1));
+	// slot = the low bits of the tiled pixel index: distinct pixels of two neighbouring tiles never share one
+	const int h = int((PARITY ? key : unsigned(pix)) & unsigned(SLOTS - 1));
 	if (live) holder[h] = lane;
 	wave_lds_sync();
 	const int w = live ? holder[h] : lane;
