@@ -502,13 +502,13 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	static_assert(SLOTS * (1 + int(sizeof(MapT)) / 4) <= 2 * 64, "hash slots and cells exceed the mark array");
 	int* holder = q.mark;
 	MapT* cell = reinterpret_cast<MapT*>(q.mark + SLOTS);
-	const unsigned key = unsigned(pix) * 2u + unsigned(bit);
+	const long long key = pix * 2 + bit;   // compared in full: maps may exceed 2^31 pixels
 	// slot = the low bits of the tiled pixel index: distinct pixels of two neighbouring tiles never share one
-	const int h = int((PARITY ? key : unsigned(pix)) & unsigned(SLOTS - 1));
+	const int h = int((PARITY ? key : pix) & (SLOTS - 1));
 	if (live) holder[h] = lane;
 	wave_lds_sync();
 	const int w = live ? holder[h] : lane;
-	const unsigned kw = __shfl(key, w);   // outside any condition: a shuffle reads nothing from inactive lanes
+	const long long kw = __shfl(key, w);   // outside any condition: a shuffle reads nothing from inactive lanes
 	const bool merged = live && w != lane && kw == key;
 	if (live && w == lane) cell[h] = v;
 	wave_lds_sync();
