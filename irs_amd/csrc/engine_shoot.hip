@@ -256,27 +256,35 @@ __device__ __forceinline__ void acc_flush(const ShootArgs& a, PixelAcc<MapT>& c)
 /*
 General triangles (crossed by more than one pixel line) are rasterised by the
 whole wave. A lane that meets one writes it, set up for per-pixel areas, into
-its wave's queue in LDS; once the queue holds more than IPM_QFLUSH triangles
+its wave's queue in LDS; once the queue holds more than FLUSH triangles
 the wave walks the pixel rows of all queued triangles, one row per lane, and
 then the pixels of those rows, one per lane. A lane holding the 100-pixel
 image of a demagnified region near a star then no longer stalls the other 63
 (those images are a tenth of the general triangles and four fifths of their
 pixels).
 */
-constexpr int IPM_QCAP = 79;                 // entries per wave
-constexpr int IPM_QFLUSH = IPM_QCAP - 64;    // an enqueue adds at most 64
+// Entries per wave: 100 (12.7 KB with the ring and the mark array) for the
+// single-map kernel, whose flushes then fill their passes better (10000^2 map:
+// 1.695 s against 1.710 s with 79); 79 (10.7 KB) for the persistent batch
+// kernel, whose 12 blocks per CU must all be resident at once (256 maps:
+// 1.664 s with 79 entries, 1.866 s with 100)
+constexpr int IPM_QCAP = 100;
+constexpr int IPM_BATCH_QCAP = 79;
 
+template <int CAP>
 struct TriQueue
 {
-	double x[3][IPM_QCAP], y[3][IPM_QCAP];       // ccw vertices relative to (X0, Y0)
-	double k[3][IPM_QCAP];                       // per edge: dx / dy (0 when dy = 0)
-	double scale[IPM_QCAP];                      // factor / triangle area
-	int X0[IPM_QCAP], Y0[IPM_QCAP];
-	int wb[IPM_QCAP];                            // box width in pixels << 1 | map (0 = total or minima, 1 = saddles)
-	int end[IPM_QCAP];                           // box rows, then running total
+	static_assert(CAP <= 128, "a ring slot keeps the queue entry in 7 bits");
+	static constexpr int FLUSH = CAP - 64;   // flush past this: an enqueue adds at most 64
+	double x[3][CAP], y[3][CAP];             // ccw vertices relative to (X0, Y0)
+	double k[3][CAP];                        // per edge: dx / dy (0 when dy = 0)
+	double scale[CAP];                       // factor / triangle area
+	int X0[CAP], Y0[CAP];
+	int wb[CAP];                             // box width in pixels << 1 | map (0 = total or minima, 1 = saddles)
+	int end[CAP];                            // box rows, then running total
 	int ring_ey[4 * 64];                         // edge-crossed pixels awaiting their exact area: entry | map bit << 7 | row << 8
 	int ring_x[4 * 64];                          //   and their map column
-	int mark[2 * 64];                            // first slot of each entry / row starting in a pass
+	int mark[4 * 64];                            // first slot of each entry / row starting in a pass; merge table
 };
 
 // a general triangle handed from the per-lane path to the wave queue
@@ -394,7 +402,8 @@ Over a row, dx p_y + dy x_i - dx y_i ranges over [A, B]; the remaining
 term -dy p_x bounds the column from the left (dy < 0) or the right (dy > 0),
 and a horizontal edge admits the whole row or none of it.
 */
-__device__ __forceinline__ void tri_queue_row(const TriQueue& q, int e, int yy, int& xl, int& xr, int& il, int& ir)
+template <class Q>
+__device__ __forceinline__ void tri_queue_row(const Q& q, int e, int yy, int& xl, int& xr, int& il, int& ir)
 {
 	const double wmax = double((q.wb[e] >> 1) - 1);
 	const double Y = double(yy);
@@ -470,12 +479,13 @@ serialise, so those lanes are summed in LDS first: each lane writes its lane
 number into the slot of its pixel (the flush's mark array is free here),
 lanes whose slot holder has the same pixel add into the slot's cell, and only
 the holders and the lanes that lost their slot to another pixel issue atomics.
-64 slots of FP32 cells, or 32 of FP64, fill the mark array's 512 bytes. On the
-10000^2 map this took the mixed kernel from 2.062 to 1.744 s; a second round
+128 slots of FP32 cells, or 64 of FP64, fill the mark array's 1 KB. On the
+10000^2 map the merge took the mixed kernel from 2.062 to 1.744 s with 64
+slots, and 128 slots took 2.5% more off (256 gained nothing); a second round
 for the lanes that lost their slot made it slower (1.92 s).
 */
-template <typename MapT, bool PARITY, bool TILED>
-__device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q, int start, int count)
+template <typename MapT, bool PARITY, bool TILED, class Q>
+__device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, Q& q, int start, int count)
 {
 	const int lane = threadIdx.x & 63;
 	int slot = start + lane;
@@ -498,8 +508,8 @@ __device__ __forceinline__ void tri_queue_exact(const ShootArgs& a, TriQueue& q,
 	const int bit = (ey >> 7) & 1;
 	const long long pix = live ? map_index<MapT, TILED>(a, X, q.Y0[e] + yy) : 0;
 	MapT v = MapT(q.scale[e] * ar);
-	constexpr int SLOTS = sizeof(MapT) == 4 ? 64 : 32;
-	static_assert(SLOTS * (1 + int(sizeof(MapT)) / 4) <= 2 * 64, "hash slots and cells exceed the mark array");
+	constexpr int SLOTS = sizeof(MapT) == 4 ? 128 : 64;
+	static_assert(SLOTS * (1 + int(sizeof(MapT)) / 4) <= 4 * 64, "hash slots and cells exceed the mark array");
 	int* holder = q.mark;
 	MapT* cell = reinterpret_cast<MapT*>(q.mark + SLOTS);
 	const long long key = pix * 2 + bit;   // compared in full: maps may exceed 2^31 pixels
@@ -530,8 +540,8 @@ the current pass marks its first slot in LDS, and a max-scan over the slots
 carries each mark forward, starting from the entry (row) the previous pass
 ended in. rh / rt: next ring slot to process / to fill.
 */
-template <typename MapT, bool PARITY, bool TILED>
-__device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q, int qn)
+template <typename MapT, bool PARITY, bool TILED, class Q>
+__device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, Q& q, int qn)
 {
 	const double factor = a.factor;
 	const int lane = threadIdx.x & 63;
@@ -697,7 +707,8 @@ __device__ __forceinline__ void tri_queue_flush(const ShootArgs& a, TriQueue& q,
 }
 
 // wave-wide: lanes with gen set append g to the queue
-__device__ __forceinline__ void tri_queue_push(TriQueue& q, int& qn, bool gen, const GenTri& g)
+template <class Q>
+__device__ __forceinline__ void tri_queue_push(Q& q, int& qn, bool gen, const GenTri& g)
 {
 	const unsigned long long m = __ballot(gen);
 	if (m == 0) return;
@@ -728,7 +739,7 @@ constexpr int IPM_WAVES = 1;
 constexpr int IPM_TILE_W = 63 * IPM_WAVES;
 constexpr int IPM_BATCH_WAVES = 1;
 constexpr int IPM_BATCH_TILE_W = 63 * IPM_BATCH_WAVES;
-constexpr int IPM_RESIDENT_WAVES_PER_CU = 16;   // 4 per SIMD: 128 VGPRs and a 10 KB wave queue
+constexpr int IPM_RESIDENT_WAVES_PER_CU = 12;   // 3 per SIMD: up to 168 VGPRs and a 12.7 KB wave queue
 constexpr int IPM_TILE_H = 64;       // lattice rows a wave walks: the shortest tile
 constexpr int IPM_TILE_H_MAX = 1024;  // and the tallest
 
@@ -748,8 +759,8 @@ static inline int ipm_tile_rows(long long ntx, long long rows, int num_cus)
 
 // One tile: this wave's 63 cells of lattice columns tile_x * 63 WAVES + wave * 63 + [0, 63),
 // cell rows [J0, J1). Every lane of the wave must call it.
-template <int PREC, bool PARITY, bool TILED, int WAVES>
-__device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long long tile_x, long long J0, long long J1)
+template <int PREC, bool PARITY, bool TILED, int WAVES, class Q>
+__device__ __forceinline__ void ipm_tile(const ShootArgs& a, Q& q, long long tile_x, long long J0, long long J1)
 {
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
@@ -790,7 +801,7 @@ __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long l
 			GenTri g;
 			const bool gen = cell_ok && ipm_triangle<PREC, PARITY, MapT, TILED>(a, acc, ax, ay, bxx, byy, cx, cy, g);
 			tri_queue_push(q, qn, gen, g);
-			if (qn > IPM_QFLUSH || (qn > 0 && t == 1 && J == J1))
+			if (qn > Q::FLUSH || (qn > 0 && t == 1 && J == J1))
 			{
 				tri_queue_flush<MapT, PARITY, TILED>(a, q, qn);
 				qn = 0;
@@ -804,8 +815,13 @@ __device__ __forceinline__ void ipm_tile(const ShootArgs& a, TriQueue& q, long l
 	acc_flush<MapT, PARITY>(a, acc);
 }
 
+// Three waves per SIMD: 136 VGPRs without spills, and 12.7 KB of LDS per wave for a
+// 100-entry queue and a 128-slot merge table. Four waves (128 VGPRs, about 25
+// spilled) ran as fast with a 64-slot table and a 79-entry queue, the most that
+// fit in 10 KB, and gained nothing from a larger table (10000^2 map: 1.698 s at
+// three waves against 1.745 s at four)
 template <int PREC, bool PARITY>
-__global__ __launch_bounds__(64 * IPM_WAVES, 4) void k_ipm_fast(ShootArgs a, long long row0, long long row1,
+__global__ __launch_bounds__(64 * IPM_WAVES, 3) void k_ipm_fast(ShootArgs a, long long row0, long long row1,
 	MLENS_TABLE_PARAMS, void* __restrict__ map, void* __restrict__ mmin, void* __restrict__ msad,
 	void* __restrict__ span, void* __restrict__ span_sad, int tile_h)
 {
@@ -818,7 +834,7 @@ __global__ __launch_bounds__(64 * IPM_WAVES, 4) void k_ipm_fast(ShootArgs a, lon
 	const long long J0 = row0 + (long long)blockIdx.y * tile_h;
 	if (J0 >= row1) return;
 	const long long J1 = min(J0 + tile_h, row1);
-	__shared__ TriQueue queues[IPM_WAVES];
+	__shared__ TriQueue<IPM_QCAP> queues[IPM_WAVES];
 	ipm_tile<PREC, PARITY, true, IPM_WAVES>(a, queues[threadIdx.x >> 6], blockIdx.x, J0, J1);
 }
 
@@ -932,7 +948,7 @@ __global__ __launch_bounds__(64 * IPM_BATCH_WAVES, 3) void k_ipm_batch(const Sho
 	const long long* __restrict__ tile_start, int ninst, int tile_h, long long total, long long stride,
 	unsigned long long* __restrict__ counter)
 {
-	__shared__ TriQueue queues[IPM_BATCH_WAVES];
+	__shared__ TriQueue<IPM_BATCH_QCAP> queues[IPM_BATCH_WAVES];
 	__shared__ long long next;
 	for (;;)
 	{
