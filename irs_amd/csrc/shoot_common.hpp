@@ -157,18 +157,28 @@ __device__ __forceinline__ Deflection deflect_leaf(const ShootArgs& a, int L, do
 		// FP32 in leaf-relative coordinates; the leaf-constant term A_1 stays FP64
 		const float zrx = float(zx - cx), zry = float(zy - cy);
 		const float4* s = reinterpret_cast<const float4*>(a.near_f) + (long long)L * MAX_NUM_STARS_DIRECT;
+		// The leaf's tables are read with scalar loads, and a wave waits for each batch before using it.
+		// Eight entries per trip instead of four halve those waits; the remainder of four (lists and
+		// polynomials are padded to fours) keeps the order of summation. Deflection alone on the
+		// 10000^2 map: 0.632 -> 0.610 s.
 		float fx = 0, fy = 0;
-		for (int k0 = 0; k0 < nn; k0 += 4)
+		auto star = [&](int k) {
+			const float4 v = s[k];
+			const float dx = zrx - v.x, dy = zry - v.y;
+			const float f = v.z * dev::rcp32(dx * dx + dy * dy);
+			fx = fmaf(f, dx, fx);
+			fy = fmaf(f, dy, fy);
+		};
+		int k0 = 0;
+		for (; k0 + 8 <= nn; k0 += 8)
 		{
 #pragma unroll
-			for (int k = k0; k < k0 + 4; k++)
-			{
-				float4 v = s[k];
-				float dx = zrx - v.x, dy = zry - v.y;
-				float f = v.z * dev::rcp32(dx * dx + dy * dy);
-				fx = fmaf(f, dx, fx);
-				fy = fmaf(f, dy, fy);
-			}
+			for (int k = k0; k < k0 + 8; k++) star(k);
+		}
+		if (k0 < nn)
+		{
+#pragma unroll
+			for (int k = k0; k < k0 + 4; k++) star(k);
 		}
 		ax = fx;
 		ay = fy;
@@ -176,16 +186,23 @@ __device__ __forceinline__ Deflection deflect_leaf(const ShootArgs& a, int L, do
 		const int SF = poly_stride_f(p);
 		const float* A = a.poly_f + (long long)L * SF * 2;
 		const float fdx = float(dzx), fdy = float(dzy);
-		for (int j0 = packed >> 16; j0 < SF; j0 += 4)
+		auto term = [&](int j) {
+			const float nx = fmaf(qx, fdx, fmaf(-qy, fdy, A[2 * j]));
+			const float ny = fmaf(qx, fdy, fmaf(qy, fdx, A[2 * j + 1]));
+			qx = nx;
+			qy = ny;
+		};
+		int j0 = packed >> 16;
+		if ((SF - j0) & 4)
 		{
 #pragma unroll
-			for (int j = j0; j < j0 + 4; j++)
-			{
-				float nx = fmaf(qx, fdx, fmaf(-qy, fdy, A[2 * j]));
-				float ny = fmaf(qx, fdy, fmaf(qy, fdx, A[2 * j + 1]));
-				qx = nx;
-				qy = ny;
-			}
+			for (int j = j0; j < j0 + 4; j++) term(j);
+			j0 += 4;
+		}
+		for (; j0 < SF; j0 += 8)
+		{
+#pragma unroll
+			for (int j = j0; j < j0 + 8; j++) term(j);
 		}
 		if (p >= 1)
 		{
