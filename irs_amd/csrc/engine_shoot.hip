@@ -941,9 +941,10 @@ struct BatchMap
 	int n1, n2, tiles_x;
 };
 
-// Three waves per SIMD (138 VGPRs, no spills): the batch's maps of 0.25-16 MB
+// Three waves per SIMD (141 VGPRs, no spills): the batch's maps of 0.25-16 MB
 // stay in the caches, and at four waves (128 VGPRs) the spills cost more than
-// the fourth wave hides (256 maps: 1.887 s at 12 blocks per CU, 2.107 s at 16)
+// the fourth wave hides (256 maps: 1.669 s at 12 blocks per CU, 1.890 s at 16).
+// Blocks take tiles from a global counter and never wait for each other.
 __global__ __launch_bounds__(64 * IPM_BATCH_WAVES, 3) void k_ipm_batch(const ShootArgs* __restrict__ args,
 	const long long* __restrict__ tile_start, int ninst, int tile_h, long long total, long long stride,
 	unsigned long long* __restrict__ counter)
