@@ -18,31 +18,30 @@ class CCF(object):
                  random_seed: int = None, write_stars: bool = None, write_critical_curves: bool = None,
                  write_caustics: bool = None, write_length_scales: bool = None, outfile_prefix: str = None,
                  verbose: int = 0):
-        '''
-        :param kappa_tot: total convergence
-        :param shear: shear
-        :param smooth_fraction: fraction of convergence due to smoothly distributed mass
-        :param kappa_star: convergence in point mass lenses
-        :param theta_star: Einstein radius of a unit mass point lens in arbitrary units
-        :param mass_function: equal, uniform, Salpeter, Kroupa or optical_depth
-        :param m_solar: solar mass in arbitrary units
-        :param m_lower: lower mass cutoff in solar mass units
-        :param m_upper: upper mass cutoff in solar mass units
-        :param rectangular: whether the star field is rectangular (True) or circular (False)
-        :param approx: whether terms for alpha_smooth are approximated (True) or exact (False)
-        :param safety_scale: ratio of the size of the star field to the size of the shooting region
-        :param num_stars: number of stars
-        :param starfile: binary star file (num_stars, rectangular, corner, theta_star, stars), or a text file
-                         of x1 x2 m per line in units where theta_star = 1; takes precedence for all star information
-        :param num_phi: number of steps of phi in [0, 2 pi]
-        :param num_branches: number of branches of phi in [0, 2 pi]
-        :param random_seed: random seed for the star field; 0 is reserved for star input files
-        :param write_stars: whether to write stars
-        :param write_critical_curves: whether to write critical curves
-        :param write_caustics: whether to write caustics
-        :param write_length_scales: whether to write magnification length scales
-        :param outfile_prefix: prefix of the output file names
-        '''
+        """Critical curves and caustics of a star field (the reference's CCF
+        class, ccf.py:7-78; same argument names, meanings and checks).
+
+        Lens: kappa_tot and shear describe the macro-model; the stellar part is
+        kappa_star, or smooth_fraction (the share of kappa_tot that is smooth
+        matter) when kappa_star is not given. theta_star is the Einstein radius
+        of a one-solar-mass lens and m_solar the solar mass, both in the caller's
+        units.
+
+        Stars: mass_function is one of equal, uniform, Salpeter, Kroupa or
+        optical_depth, cut to [m_lower, m_upper] solar masses. The field is a
+        rectangle or a disk (rectangular), safety_scale times as large as the
+        region it must cover, with num_stars stars drawn from random_seed (0 is
+        kept for fields read from a file). A starfile, binary (count, shape,
+        corner, theta_star, then the stars) or text (x1 x2 m per line with
+        theta_star = 1), replaces every star setting. approx selects the
+        approximate terms of the smooth-matter deflection.
+
+        Curves: num_phi steps of the phase over [0, 2 pi], split into
+        num_branches branches (num_phi even and a multiple of 2 * num_branches).
+
+        Output: write_stars, write_critical_curves, write_caustics and
+        write_length_scales choose the files written under outfile_prefix.
+        """
         self.lib = lib_ccf.lib
         self.obj = self.lib.init()
         self.verbose = verbose
