@@ -1365,8 +1365,8 @@ __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long
 	a.map = map;
 	a.mmin = mmin;
 	a.msad = msad;
+	static_assert(IRS_THREADS == 64, "the window is placed from one wave's reduction of its first row");
 	__shared__ int win_counts[IRS_WX * IRS_WY];
-	__shared__ int box[4];
 	const long long i = (long long)blockIdx.x * IRS_THREADS + threadIdx.x;
 	const long long J0 = row0 + (long long)blockIdx.y * tile_h;
 	if (J0 >= row1) return;
@@ -1376,11 +1376,6 @@ __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long
 	LeafCache cache;
 	unsigned long long got = 0;
 	for (int k = threadIdx.x; k < IRS_WX * IRS_WY; k += IRS_THREADS) win_counts[k] = 0;
-	if (threadIdx.x == 0)
-	{
-		box[0] = box[1] = 0x7fffffff;
-		box[2] = box[3] = -0x7fffffff;
-	}
 	__syncthreads();
 
 	// register cache: (x, y) of the two most recent pixels and their counts
@@ -1395,7 +1390,7 @@ __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long
 		const bool hit = ok && !isnan(d.wx) && !isnan(d.wy) && irs_pixel(a, d.wx, d.wy, px, py);
 		if (j == J0)
 		{
-			// window centred on the bounding box of the block's first row
+			// window centred on the bounding box of the block's (the wave's) first row
 			int mnx = hit ? px : 0x7fffffff, mny = hit ? py : 0x7fffffff;
 			int mxx = hit ? px : -0x7fffffff, mxy = hit ? py : -0x7fffffff;
 			for (int off = 32; off > 0; off >>= 1)
@@ -1405,18 +1400,10 @@ __global__ __launch_bounds__(IRS_THREADS) void k_irs_fast(ShootArgs a, long long
 				mxx = max(mxx, __shfl_xor(mxx, off));
 				mxy = max(mxy, __shfl_xor(mxy, off));
 			}
-			if ((threadIdx.x & 63) == 0)
+			if (mnx <= mxx)
 			{
-				atomicMin(&box[0], mnx);
-				atomicMin(&box[1], mny);
-				atomicMax(&box[2], mxx);
-				atomicMax(&box[3], mxy);
-			}
-			__syncthreads();
-			if (box[0] <= box[2])
-			{
-				win.x0 = (box[0] + box[2]) / 2 - IRS_WX / 2;
-				win.y0 = (box[1] + box[3]) / 2 - IRS_WY / 2;
+				win.x0 = (mnx + mxx) / 2 - IRS_WX / 2;
+				win.y0 = (mny + mxy) / 2 - IRS_WY / 2;
 			}
 			else
 			{
